@@ -1,0 +1,138 @@
+"""Benchmark: horizon + SVF throughput on one MI355X.
+
+Workload (default): 1 km ICON-like grid over a 600 x 400 km box (553,200
+cells, 1.66M grid_type-1 triangles), alpine-like synthetic terrain, nhori=24,
+dist_search=40 km, ray_org_elev=0.2 m, refine_factor=1, svf_type=2.
+
+One step = one full device pass of the hot path with the FP32 scene and cell
+frames already resident in HBM: LBVH build, Morton cell ordering, the
+elevation search of every cell and the fused sector mean / SVF, outputs left
+in HBM.  value = cells per second over the timed steps.
+
+Prints ONE JSON line (rank 0).  The host-CPU baseline (the oracle's C++ port
+of the reference walk, all host threads, time-budgeted sample) is timed in
+the same run unless --no-cpu.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", default="c2", choices=["c1", "c2"])
+    ap.add_argument("--mode", default="walk", choices=["walk", "climb"])
+    ap.add_argument("--leaf", type=int, default=0)
+    ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the "
+              "horizon search shards by cell ranges with a replicated scene and is not "
+              "run across GPUs here.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    from horayzon_extpar_amd import synthetic
+    from horayzon_extpar_amd.horizon_svf import ResidentDomain
+
+    mode = {"walk": 0, "climb": 1}[args.mode]
+    dom_in = synthetic.CONFIGS[args.config]()
+    nhori, dist, rf, svf = 24, 40000.0, 1, 2
+    print(f"[bench] workload {args.config}: ncell={dom_in.ncell} nvert={dom_in.nvert}",
+          file=sys.stderr, flush=True)
+    opts = dict(search_mode=mode, leaf_size=args.leaf or None, block_size=args.block or None)
+    with ResidentDomain(*dom_in.args(), grid_type=1, ray_org_elev=0.2) as dom:
+        cst = dom.create_stats
+        for _ in range(args.warmup):
+            dom.run(nhori, dist, rf, svf, **opts)
+        # instrumented pass (untimed): node visits and triangle tests per ray
+        cnt = dom.run(nhori, dist, rf, svf, collect_counters=1, **opts)
+        print(f"[bench] counters: rays={cnt['num_rays']} nodes={cnt['node_visits']} "
+              f"tris={cnt['tri_tests']} depth={cnt['bvh_depth']}", file=sys.stderr, flush=True)
+        t_trace, t_bvh, t_ord, rays = [], [], [], 0
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            st = dom.run(nhori, dist, rf, svf, **opts)
+            t_trace.append(st["t_trace_ms"])
+            t_bvh.append(st["t_bvh_ms"])
+            t_ord.append(st["t_order_ms"])
+            rays = st["num_rays"]
+        t1 = time.perf_counter()
+        h, s = dom.fetch()
+    ncell = dom_in.ncell
+    ms_step = (t1 - t0) * 1e3 / args.steps
+    value = ncell / (ms_step * 1e-3)
+    tr = float(np.mean(t_trace))
+    n_node = cnt["node_visits"] / cnt["num_rays"]
+    n_tri = cnt["tri_tests"] / cnt["num_rays"]
+    b_ray = 64.0 * n_node + 36.0 * n_tri
+    bytes_launch = rays * b_ray + ncell * (64 + 8 * (nhori + 1))
+    achieved = bytes_launch / (tr * 1e-3) / 1e9
+    out = {
+        "metric": "horizon+SVF cells/s @nhori=24",
+        "value": value,
+        "unit": "cells/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 ray tests, f64 walk state",
+        "data": "synthetic (seeded ICON-like grid and terrain)",
+        "config": {"workload": f"{args.config}: 600x400 km @1 km ICON-like grid, "
+                               f"{ncell} cells, {cnt['num_triangles']} triangles (grid_type 1)",
+                   "num_hori": nhori, "dist_search_m": dist, "refine_factor": rf,
+                   "svf_type": svf, "search": args.mode},
+        "rays_per_s": rays / (tr * 1e-3),
+        "rays_per_cell_sector": rays / (ncell * nhori),
+        "ms_trace": tr, "ms_bvh": float(np.mean(t_bvh)), "ms_order": float(np.mean(t_ord)),
+        "ms_prologue_host": cst.t_prologue_ms, "ms_upload": cst.t_upload_ms,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS,
+                     "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS, "traffic": None,
+                     "kernel": "k_trace", "bytes_per_ray": b_ray,
+                     "nodes_per_ray": n_node, "tris_per_ray": n_tri},
+        "horizon_checksum": float(np.sum(h)), "skyview_mean": float(np.mean(s)),
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(dom_in, args.cpu_budget, args.cpu_threads)
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline(dom_in, budget_s, threads):
+    """The oracle (C++ port of the reference walk, FP32 CPU BVH, std::thread
+    pool with dynamic chunks) on a time-budgeted contiguous block of cells at
+    the domain centre."""
+    import oracle as orc
+
+    nt = threads or min(os.cpu_count() or 1, 16)
+    n = dom_in.ncell
+    b = n // 2 - n // 8
+    _, _, st = orc.horizon_svf(dom_in, grid_type=1, nthreads=nt, cell_begin=b,
+                               cell_end=b + n // 4, time_budget_s=budget_s)
+    cells = st.cells_done
+    return {"value": cells / st.t_trace, "unit": "cells/s", "cores": nt, "kind": "port",
+            "sample": f"{cells} cells (contiguous block at the domain centre, "
+                      f"{st.t_trace:.1f} s of tracing); setup {st.t_setup:.1f} s excluded",
+            "rays_per_s": st.num_rays / st.t_trace}
+
+
+if __name__ == "__main__":
+    main()

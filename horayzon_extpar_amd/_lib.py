@@ -1,0 +1,122 @@
+"""ctypes binding of libhorayzon.so (C-ABI of include/horayzon.h).
+
+The library is loaded from the package directory; a missing library is an
+error (there is no fallback implementation).
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+
+from .build import library_path
+
+_lib = None
+
+c_double_p = ctypes.POINTER(ctypes.c_double)
+c_int_p = ctypes.POINTER(ctypes.c_int)
+
+HZ_OK = 0
+HZ_EINVAL = -1
+HZ_EDEVICE = -2
+HZ_ENOMEM = -3
+HZ_EKERNEL = -4
+
+HZ_MODE_WALK = 0
+HZ_MODE_CLIMB = 1
+
+
+class HzOptions(ctypes.Structure):
+    _fields_ = [("search_mode", ctypes.c_int), ("leaf_size", ctypes.c_int),
+                ("block_size", ctypes.c_int), ("collect_counters", ctypes.c_int),
+                ("cell_order", ctypes.c_int), ("verbose", ctypes.c_int),
+                ("reserved", ctypes.c_int * 10)]
+
+
+class HzStats(ctypes.Structure):
+    _fields_ = [("num_rays", ctypes.c_longlong), ("node_visits", ctypes.c_longlong),
+                ("tri_tests", ctypes.c_longlong), ("num_triangles", ctypes.c_longlong),
+                ("num_mesh_vertices", ctypes.c_longlong), ("bvh_nodes", ctypes.c_longlong),
+                ("bvh_depth", ctypes.c_int), ("kernel_error", ctypes.c_int),
+                ("t_prologue_ms", ctypes.c_double), ("t_upload_ms", ctypes.c_double),
+                ("t_bvh_ms", ctypes.c_double), ("t_order_ms", ctypes.c_double),
+                ("t_trace_ms", ctypes.c_double), ("t_download_ms", ctypes.c_double),
+                ("t_total_ms", ctypes.c_double), ("lon0", ctypes.c_double),
+                ("lat0", ctypes.c_double), ("reserved", ctypes.c_double * 8)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
+# (name, restype, argtypes) of every exported entry point in include/horayzon.h
+_ARR6 = [c_double_p] * 5 + [c_int_p]
+SIGNATURES = [
+    ("horizon_svf_comp", None,
+     _ARR6 + [c_double_p, c_double_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+              ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int]),
+    ("horizon_svf_comp_ex", ctypes.c_int,
+     _ARR6 + [c_double_p, c_double_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+              ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+              ctypes.POINTER(HzOptions), ctypes.POINTER(HzStats)]),
+    ("hz_last_error", ctypes.c_char_p, []),
+    ("hz_default_options", None, [ctypes.POINTER(HzOptions)]),
+    ("hz_domain_create", ctypes.c_int,
+     _ARR6 + [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+              ctypes.POINTER(HzOptions), ctypes.POINTER(ctypes.c_void_p),
+              ctypes.POINTER(HzStats)]),
+    ("hz_domain_run", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+      ctypes.POINTER(HzOptions), ctypes.POINTER(HzStats)]),
+    ("hz_domain_fetch", ctypes.c_int,
+     [ctypes.c_void_p, c_double_p, c_double_p, ctypes.POINTER(HzStats)]),
+    ("hz_domain_destroy", None, [ctypes.c_void_p]),
+    ("hz_scene_export", ctypes.c_int,
+     _ARR6 + [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+              ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong),
+              ctypes.POINTER(ctypes.c_float), c_int_p, c_double_p, c_double_p,
+              ctypes.POINTER(ctypes.c_float), c_double_p]),
+    ("hz_domain_num_cell", ctypes.c_int, [ctypes.c_void_p]),
+    ("hz_domain_num_triangles", ctypes.c_longlong, [ctypes.c_void_p]),
+]
+
+
+def lib():
+    """Load libhorayzon.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        path = library_path()
+        if not os.path.exists(path):
+            raise RuntimeError(
+                f"{path} not found: build it with horayzon_extpar_amd.build_library() "
+                "(hipcc, gfx950); there is no CPU fallback")
+        L = ctypes.CDLL(path)
+        for name, res, args in SIGNATURES:
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def last_error() -> str:
+    return lib().hz_last_error().decode("utf-8", "replace")
+
+
+def default_options(**kw) -> HzOptions:
+    o = HzOptions()
+    lib().hz_default_options(ctypes.byref(o))
+    for k, v in kw.items():
+        if v is not None:
+            setattr(o, k, v)
+    return o
+
+
+class HzError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"libhorayzon error {code}: {msg}")
+        self.code = code
+
+
+def check(rc: int):
+    if rc != HZ_OK:
+        raise HzError(rc, last_error())

@@ -1,0 +1,431 @@
+// C-ABI of libhorayzon.so (declared in include/horayzon.h).
+//
+// horizon_svf_comp keeps the reference signature and output layout
+// (mo_lradtopo_horayzon.h:4-11, mo_lradtopo_horayzon.cpp:562-785).  The
+// reference has no C-side validation and no status; this layer validates the
+// same ranges the reference's Python binding checks (horizon_svf.pyx:79-103),
+// returns a status from the _ex / hz_domain_* entry points and fills the
+// outputs with NaN on failure.  There is no global mutable state besides the
+// per-thread error message, so calls are reentrant (each call owns its
+// stream and buffers).
+
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <limits>
+
+#include "../../include/horayzon.h"
+#include "hz_internal.h"
+
+using namespace hz;
+
+namespace {
+
+thread_local std::string g_err;
+
+double now_ms() {
+    return std::chrono::duration<double, std::milli>(
+               std::chrono::steady_clock::now().time_since_epoch())
+        .count();
+}
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+bool verbose_env() {
+    const char* v = std::getenv("HZ_VERBOSE");
+    return v && v[0] && v[0] != '0';
+}
+
+void fill_nan(double* horizon, double* skyview, int num_hori, int num_cell) {
+    const double nan = std::numeric_limits<double>::quiet_NaN();
+    if (horizon && num_hori > 0 && num_cell > 0)
+        for (size_t i = 0; i < (size_t)num_hori * (size_t)num_cell; i++) horizon[i] = nan;
+    if (skyview && num_cell > 0)
+        for (int i = 0; i < num_cell; i++) skyview[i] = nan;
+}
+
+int check_grid_args(const int* cov, int num_cell, int num_vertex, int grid_type,
+                    double ray_org_elev) {
+    if (num_cell < 1) return fail(HZ_EINVAL, "num_cell must be >= 1");
+    if (num_vertex < 0) return fail(HZ_EINVAL, "num_vertex must be >= 0");
+    if (grid_type < 0 || grid_type > 1)
+        return fail(HZ_EINVAL, "'grid_type' must be in the range [0, 1]");
+    if (!(ray_org_elev >= 0.1)) return fail(HZ_EINVAL, "Minimal allowed value for 'ray_org_elev' is 0.1 m");
+    for (size_t k = 0; k < (size_t)6 * (size_t)num_vertex; k++) {
+        int c = cov[k];
+        if (!((c >= 0 && c <= num_cell - 1) || c == -2))
+            return fail(HZ_EINVAL, "Indices of 'cells_of_vertex' out of range");
+    }
+    return 0;
+}
+
+int check_run_args(int num_hori, double dist_search, int refine_factor, int svf_type) {
+    if (num_hori < 4 || num_hori > 1440)
+        return fail(HZ_EINVAL, "'num_hori' must be in the range [4, 1440]");
+    if (!(dist_search >= 1000.0 && dist_search <= 500000.0))
+        return fail(HZ_EINVAL, "'dist_search' must be in the range [1'000, 500'000] m");
+    if (refine_factor < 1 || refine_factor > 50)
+        return fail(HZ_EINVAL, "'refine_factor' must be in the range [1, 50]");
+    if (svf_type < 0 || svf_type > 2)
+        return fail(HZ_EINVAL, "'svf_type' must be in the range [0, 2]");
+    return 0;
+}
+
+}  // namespace
+
+struct hz_domain {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int ncell = 0;
+    int grid_type = 1;
+    long long ntri = 0;
+    long long nmeshvert = 0;
+    double lon0 = 0, lat0 = 0;
+    float* d_vert = nullptr;
+    int* d_tri = nullptr;
+    CellFrame* d_cells = nullptr;
+    int* d_order = nullptr;
+    BvhNode* d_nodes = nullptr;
+    float4* d_tris = nullptr;
+    double* d_horizon = nullptr;
+    double* d_skyview = nullptr;
+    int out_nhori = 0;  // horizon rows held by d_horizon
+    int last_nhori = 0; // rows written by the last run
+    TraceCounters* d_cnt = nullptr;
+    LbvhWork work;
+    DeviceBvh bvh;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    bool ran = false;
+};
+
+extern "C" {
+
+const char* hz_last_error(void) { return g_err.c_str(); }
+
+void hz_default_options(hz_options* o) {
+    std::memset(o, 0, sizeof(*o));
+    o->search_mode = HZ_MODE_WALK;
+    o->leaf_size = 4;
+    o->block_size = 128;
+    o->collect_counters = 0;
+    o->cell_order = 1;
+    o->verbose = 0;
+}
+
+void hz_domain_destroy(hz_domain* d) {
+    if (!d) return;
+    (void)hipSetDevice(d->device);
+    if (d->stream) (void)hipStreamSynchronize(d->stream);
+    void* bufs[] = {d->d_vert,   d->d_tri,     d->d_cells,   d->d_order, d->d_nodes,
+                    d->d_tris,   d->d_horizon, d->d_skyview, d->d_cnt};
+    for (void* b : bufs)
+        if (b) (void)hipFree(b);
+    lbvh_free(d->work);
+    for (auto& e : d->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (d->stream) (void)hipStreamDestroy(d->stream);
+    delete d;
+}
+
+int hz_domain_num_cell(const hz_domain* d) { return d ? d->ncell : 0; }
+
+int hz_scene_export(const double* clon, const double* clat, const double* hsurf,
+                    const double* vlon, const double* vlat, const int* cov, int num_cell,
+                    int num_vertex, int grid_type, double ray_org_elev, long long* num_triangles,
+                    long long* num_mesh_vertices, float* vert32, int* tri, double* frame_n,
+                    double* frame_north, float* org, double* lon0_lat0) {
+    g_err.clear();
+    if (!clon || !clat || !hsurf || (num_vertex > 0 && (!vlon || !vlat || !cov)))
+        return fail(HZ_EINVAL, "NULL input array");
+    int rc = check_grid_args(cov, num_cell, num_vertex, grid_type, ray_org_elev);
+    if (rc) return rc;
+    HostScene hs;
+    std::string err;
+    if (prologue(clon, clat, hsurf, vlon, vlat, cov, num_cell, num_vertex, grid_type,
+                 ray_org_elev, hs, err))
+        return fail(HZ_EINVAL, err);
+    if (num_triangles) *num_triangles = (long long)hs.tri.size() / 3;
+    if (num_mesh_vertices) *num_mesh_vertices = hs.num_mesh_vertices;
+    if (vert32) std::memcpy(vert32, hs.vert32.data(), hs.vert32.size() * sizeof(float));
+    if (tri) std::memcpy(tri, hs.tri.data(), hs.tri.size() * sizeof(int));
+    for (int i = 0; i < num_cell; i++) {
+        const CellFrame& f = hs.frame[i];
+        for (int a = 0; a < 3; a++) {
+            if (frame_n) frame_n[3 * i + a] = f.n[a];
+            if (frame_north) frame_north[3 * i + a] = f.north[a];
+            if (org) org[3 * i + a] = f.org[a];
+        }
+    }
+    if (lon0_lat0) {
+        lon0_lat0[0] = hs.lon0;
+        lon0_lat0[1] = hs.lat0;
+    }
+    return HZ_OK;
+}
+long long hz_domain_num_triangles(const hz_domain* d) { return d ? d->ntri : 0; }
+
+#define DCHK(call)                                                                     \
+    do {                                                                               \
+        hipError_t e_ = (call);                                                        \
+        if (e_ != hipSuccess) {                                                        \
+            hz_domain_destroy(d);                                                      \
+            return fail(e_ == hipErrorOutOfMemory ? HZ_ENOMEM : HZ_EDEVICE,           \
+                        std::string(#call) + ": " + hipGetErrorString(e_));            \
+        }                                                                              \
+    } while (0)
+
+int hz_domain_create(const double* clon, const double* clat, const double* hsurf,
+                     const double* vlon, const double* vlat, const int* cov, int num_cell,
+                     int num_vertex, int grid_type, double ray_org_elev, const hz_options* opt,
+                     hz_domain** out, hz_stats* stats) {
+    (void)opt;
+    g_err.clear();
+    if (!out) return fail(HZ_EINVAL, "out is NULL");
+    *out = nullptr;
+    if (!clon || !clat || !hsurf || (num_vertex > 0 && (!vlon || !vlat || !cov)))
+        return fail(HZ_EINVAL, "NULL input array");
+    int rc = check_grid_args(cov, num_cell, num_vertex, grid_type, ray_org_elev);
+    if (rc) return rc;
+    double t0 = now_ms();
+    HostScene hs;
+    std::string err;
+    rc = prologue(clon, clat, hsurf, vlon, vlat, cov, num_cell, num_vertex, grid_type,
+                  ray_org_elev, hs, err);
+    if (rc) return fail(HZ_EINVAL, err);
+    double t1 = now_ms();
+    hz_domain* d = new hz_domain();
+    {
+        int ndev = 0;
+        hipError_t e = hipGetDeviceCount(&ndev);
+        if (e != hipSuccess || ndev < 1) {
+            delete d;
+            return fail(HZ_EDEVICE, "no HIP device available");
+        }
+    }
+    DCHK(hipGetDevice(&d->device));
+    DCHK(hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking));
+    for (auto& e : d->ev) DCHK(hipEventCreate(&e));
+    d->ncell = num_cell;
+    d->grid_type = grid_type;
+    d->ntri = (long long)hs.tri.size() / 3;
+    d->nmeshvert = hs.num_mesh_vertices;
+    d->lon0 = hs.lon0;
+    d->lat0 = hs.lat0;
+    size_t nv = hs.vert32.size(), nt = hs.tri.size();
+    DCHK(hipMalloc(&d->d_vert, std::max<size_t>(nv, 3) * sizeof(float)));
+    DCHK(hipMalloc(&d->d_tri, std::max<size_t>(nt, 3) * sizeof(int)));
+    DCHK(hipMalloc(&d->d_cells, (size_t)num_cell * sizeof(CellFrame)));
+    DCHK(hipMalloc(&d->d_order, (size_t)num_cell * sizeof(int)));
+    DCHK(hipMalloc(&d->d_nodes, std::max<long long>(1, d->ntri - 1) * sizeof(BvhNode)));
+    DCHK(hipMalloc(&d->d_tris, std::max<long long>(1, d->ntri) * 3 * sizeof(float4)));
+    DCHK(hipMalloc(&d->d_skyview, (size_t)num_cell * sizeof(double)));
+    DCHK(hipMalloc(&d->d_cnt, sizeof(TraceCounters)));
+    if (nv) DCHK(hipMemcpyAsync(d->d_vert, hs.vert32.data(), nv * sizeof(float), hipMemcpyHostToDevice, d->stream));
+    if (nt) DCHK(hipMemcpyAsync(d->d_tri, hs.tri.data(), nt * sizeof(int), hipMemcpyHostToDevice, d->stream));
+    DCHK(hipMemcpyAsync(d->d_cells, hs.frame.data(), (size_t)num_cell * sizeof(CellFrame),
+                        hipMemcpyHostToDevice, d->stream));
+    if (lbvh_reserve(d->work, (int)std::max<long long>(std::max<long long>(d->ntri, num_cell), 1), err)) {
+        hz_domain_destroy(d);
+        return fail(HZ_ENOMEM, err);
+    }
+    DCHK(hipStreamSynchronize(d->stream));
+    double t2 = now_ms();
+    if (stats) {
+        std::memset(stats, 0, sizeof(*stats));
+        stats->num_triangles = d->ntri;
+        stats->num_mesh_vertices = d->nmeshvert;
+        stats->t_prologue_ms = t1 - t0;
+        stats->t_upload_ms = t2 - t1;
+        stats->lon0 = d->lon0;
+        stats->lat0 = d->lat0;
+    }
+    *out = d;
+    return HZ_OK;
+}
+
+#undef DCHK
+#define RCHK(call)                                                                     \
+    do {                                                                               \
+        hipError_t e_ = (call);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return fail(e_ == hipErrorOutOfMemory ? HZ_ENOMEM : HZ_EDEVICE,           \
+                        std::string(#call) + ": " + hipGetErrorString(e_));            \
+    } while (0)
+
+int hz_domain_run(hz_domain* d, int num_hori, double dist_search, int refine_factor,
+                  int svf_type, const hz_options* opt, hz_stats* stats) {
+    g_err.clear();
+    if (!d) return fail(HZ_EINVAL, "domain is NULL");
+    int rc = check_run_args(num_hori, dist_search, refine_factor, svf_type);
+    if (rc) return rc;
+    hz_options o;
+    if (opt)
+        o = *opt;
+    else
+        hz_default_options(&o);
+    if (o.search_mode != HZ_MODE_WALK && o.search_mode != HZ_MODE_CLIMB)
+        return fail(HZ_EINVAL, "unknown search_mode");
+    int block = o.block_size ? o.block_size : 128;
+    if (block != 64 && block != 128 && block != 256)
+        return fail(HZ_EINVAL, "block_size must be 64, 128 or 256");
+    int leaf = o.leaf_size ? o.leaf_size : 4;
+    if (leaf < 1 || leaf > kMaxLeafSize) return fail(HZ_EINVAL, "leaf_size must be in [1, 8]");
+    bool verbose = o.verbose || verbose_env();
+    RCHK(hipSetDevice(d->device));
+    double t0 = now_ms();
+    if (num_hori > d->out_nhori) {
+        if (d->d_horizon) RCHK(hipFree(d->d_horizon));
+        d->d_horizon = nullptr;
+        RCHK(hipMalloc(&d->d_horizon, (size_t)num_hori * d->ncell * sizeof(double)));
+        d->out_nhori = num_hori;
+    }
+    std::string err;
+    hipStream_t st = d->stream;
+    RCHK(hipMemsetAsync(d->d_cnt, 0, sizeof(TraceCounters), st));
+    RCHK(hipEventRecord(d->ev[0], st));
+    d->bvh.nodes = d->d_nodes;
+    d->bvh.tris = d->d_tris;
+    rc = lbvh_build(d->work, d->d_vert, d->d_tri, (int)d->ntri, leaf, st, d->bvh,
+                    o.collect_counters != 0, err);
+    if (rc) return fail(rc == -3 ? HZ_ENOMEM : HZ_EDEVICE, err);
+    RCHK(hipEventRecord(d->ev[1], st));
+    const int* order = nullptr;
+    if (o.cell_order) {
+        rc = order_cells(d->work, d->d_cells, d->ncell, d->d_order, st, err);
+        if (rc) return fail(HZ_EDEVICE, err);
+        order = d->d_order;
+    }
+    RCHK(hipEventRecord(d->ev[2], st));
+    WalkParams p = make_walk_params(num_hori, refine_factor, svf_type, dist_search, d->ncell);
+    rc = launch_trace(d->bvh, d->d_cells, order, p, o.search_mode, block, o.collect_counters != 0,
+                      d->d_horizon, d->d_skyview, d->d_cnt, st, err);
+    if (rc) return fail(HZ_EDEVICE, err);
+    RCHK(hipEventRecord(d->ev[3], st));
+    TraceCounters c;
+    RCHK(hipMemcpyAsync(&c, d->d_cnt, sizeof(c), hipMemcpyDeviceToHost, st));
+    RCHK(hipStreamSynchronize(st));
+    float ms_bvh = 0, ms_ord = 0, ms_trace = 0;
+    RCHK(hipEventElapsedTime(&ms_bvh, d->ev[0], d->ev[1]));
+    RCHK(hipEventElapsedTime(&ms_ord, d->ev[1], d->ev[2]));
+    RCHK(hipEventElapsedTime(&ms_trace, d->ev[2], d->ev[3]));
+    double t1 = now_ms();
+    d->last_nhori = num_hori;
+    d->ran = c.error == 0;
+    if (verbose) {
+        std::fprintf(stderr, "Building bounding volume hierarchy (BVH): %.2f ms\n", ms_bvh);
+        std::fprintf(stderr, "Ray tracing: %.2f ms\n", ms_trace);
+        std::fprintf(stderr, "Number of rays shot: %llu\n", (unsigned long long)c.rays);
+        std::fprintf(stderr, "Average number of rays per cell and azimuth sector: %.2f\n",
+                     (double)c.rays / ((double)d->ncell * num_hori));
+    }
+    if (stats) {
+        stats->num_rays = (long long)c.rays;
+        stats->node_visits = (long long)c.nodes;
+        stats->tri_tests = (long long)c.tris;
+        stats->num_triangles = d->ntri;
+        stats->num_mesh_vertices = d->nmeshvert;
+        stats->bvh_nodes = d->bvh.num_nodes;
+        stats->bvh_depth = d->bvh.depth;
+        stats->kernel_error = (int)c.error;
+        stats->t_bvh_ms = ms_bvh;
+        stats->t_order_ms = ms_ord;
+        stats->t_trace_ms = ms_trace;
+        stats->t_total_ms = t1 - t0;
+        stats->lon0 = d->lon0;
+        stats->lat0 = d->lat0;
+    }
+    if (c.error) return fail(HZ_EKERNEL, "trace kernel exceeded its traversal stack or walk limit");
+    return HZ_OK;
+}
+
+int hz_domain_fetch(hz_domain* d, double* horizon, double* skyview, hz_stats* stats) {
+    g_err.clear();
+    if (!d) return fail(HZ_EINVAL, "domain is NULL");
+    if (!d->ran) return fail(HZ_EINVAL, "no successful hz_domain_run to fetch");
+    RCHK(hipSetDevice(d->device));
+    double t0 = now_ms();
+    if (horizon)
+        RCHK(hipMemcpyAsync(horizon, d->d_horizon, (size_t)d->last_nhori * d->ncell * sizeof(double),
+                            hipMemcpyDeviceToHost, d->stream));
+    if (skyview)
+        RCHK(hipMemcpyAsync(skyview, d->d_skyview, (size_t)d->ncell * sizeof(double),
+                            hipMemcpyDeviceToHost, d->stream));
+    RCHK(hipStreamSynchronize(d->stream));
+    if (stats) stats->t_download_ms = now_ms() - t0;
+    return HZ_OK;
+}
+
+int horizon_svf_comp_ex(const double* clon, const double* clat, const double* hsurf,
+                        const double* vlon, const double* vlat, const int* cov, double* horizon,
+                        double* skyview, int num_cell, int num_vertex, int num_hori,
+                        int grid_type, double dist_search, double ray_org_elev,
+                        int refine_factor, int svf_type, const hz_options* opt,
+                        hz_stats* stats) {
+    double t0 = now_ms();
+    g_err.clear();
+    if (!horizon || !skyview) return fail(HZ_EINVAL, "NULL output array");
+    int rc = check_run_args(num_hori, dist_search, refine_factor, svf_type);
+    if (rc) {
+        fill_nan(horizon, skyview, num_hori, num_cell);
+        return rc;
+    }
+    hz_options o;
+    if (opt)
+        o = *opt;
+    else
+        hz_default_options(&o);
+    bool verbose = o.verbose || verbose_env();
+    if (verbose) {
+        std::fprintf(stderr, "-------------------------------------------------------------------------------\n");
+        std::fprintf(stderr, "Horizon and SVF computation with HIP on gfx950\n");
+        std::fprintf(stderr, "-------------------------------------------------------------------------------\n");
+    }
+    hz_stats local;
+    hz_stats* s = stats ? stats : &local;
+    std::memset(s, 0, sizeof(*s));
+    hz_domain* d = nullptr;
+    rc = hz_domain_create(clon, clat, hsurf, vlon, vlat, cov, num_cell, num_vertex, grid_type,
+                          ray_org_elev, &o, &d, s);
+    if (rc) {
+        fill_nan(horizon, skyview, num_hori, num_cell);
+        return rc;
+    }
+    double tp = s->t_prologue_ms, tu = s->t_upload_ms;
+    if (verbose) {
+        std::fprintf(stderr, "Number of triangles in mesh: %lld\n", d->ntri);
+        std::fprintf(stderr, "Origin of ENU coordinate system: %.4f deg lat, %.4f deg lon\n",
+                     d->lat0 * 180.0 / 3.14159265358979323846, d->lon0 * 180.0 / 3.14159265358979323846);
+    }
+    rc = hz_domain_run(d, num_hori, dist_search, refine_factor, svf_type, &o, s);
+    if (rc == 0) rc = hz_domain_fetch(d, horizon, skyview, s);
+    std::string keep = g_err;
+    hz_domain_destroy(d);
+    g_err = keep;
+    s->t_prologue_ms = tp;
+    s->t_upload_ms = tu;
+    s->t_total_ms = now_ms() - t0;
+    if (rc) {
+        fill_nan(horizon, skyview, num_hori, num_cell);
+        return rc;
+    }
+    return HZ_OK;
+}
+
+void horizon_svf_comp(double* clon, double* clat, double* hsurf, double* vlon, double* vlat,
+                      int* cells_of_vertex, double* horizon, double* skyview, int num_cell,
+                      int num_vertex, int num_hori, int grid_type, double dist_search,
+                      double ray_org_elev, int refine_factor, int svf_type) {
+    int rc = horizon_svf_comp_ex(clon, clat, hsurf, vlon, vlat, cells_of_vertex, horizon, skyview,
+                                 num_cell, num_vertex, num_hori, grid_type, dist_search,
+                                 ray_org_elev, refine_factor, svf_type, nullptr, nullptr);
+    if (rc) std::fprintf(stderr, "horizon_svf_comp failed (%d): %s\n", rc, g_err.c_str());
+}
+
+}  // extern "C"

@@ -1,0 +1,143 @@
+// Internal declarations shared by the host prologue, the LBVH builder, the
+// trace kernels and the C-ABI layer of libhorayzon.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace hz {
+
+// ---------------------------------------------------------------------------
+// Device-side records
+// ---------------------------------------------------------------------------
+
+// Per-cell ray frame: unit sphere normal n and unit north direction (FP64,
+// reference mo_lradtopo_horayzon.cpp:693-712) and the FP32 ray origin
+// (:715-720).  64 bytes, one record per cell.
+struct alignas(16) CellFrame {
+    double n[3];
+    double north[3];
+    float org[3];
+    int pad;
+};
+static_assert(sizeof(CellFrame) == 64, "CellFrame layout");
+
+// BVH2 interior node: the conservative boxes of both children and the two
+// child references, 64 bytes = four float4 loads.
+//   ref >= 0 : interior node index
+//   ref <  0 : leaf; ~ref = first | ((count - 1) << 28), triangles in leaf order
+struct alignas(16) BvhNode {
+    float lx0, lx1, hx0, hx1;
+    float ly0, ly1, hy0, hy1;
+    float lz0, lz1, hz0, hz1;
+    int r0, r1, pad0, pad1;
+};
+static_assert(sizeof(BvhNode) == 64, "BvhNode layout");
+
+constexpr int kLeafFirstBits = 28;
+constexpr int kLeafFirstMask = (1 << kLeafFirstBits) - 1;
+constexpr int kMaxLeafSize = 8;
+
+// Triangles in leaf order: 3 float4 per triangle
+//   {v0.x, v0.y, v0.z, v1.x}, {v1.y, v1.z, v2.x, v2.y}, {v2.z, id, 0, 0}
+struct DeviceBvh {
+    BvhNode* nodes = nullptr;
+    float4* tris = nullptr;
+    int num_tri = 0;
+    int num_nodes = 0;  // interior nodes (>= 1 whenever num_tri >= 1)
+    int depth = 0;      // deepest interior level, root = 0
+};
+
+// Constants of the elevation search (reference :577-587, :651-668).
+struct WalkParams {
+    double hori_acc;        // 0.25 deg [rad]
+    double elev_thresh;     // -85 deg [rad]
+    double azim_sin, azim_cos;  // of 360 deg / (nhori * rf)
+    double elev_sin_2ha, elev_cos_2ha;
+    double shift_sin, shift_cos;  // of -azim_shift
+    float tfar;             // (float) dist_search
+    int num_hori;
+    int refine_factor;
+    int svf_type;
+    int ncell;
+};
+
+// ---------------------------------------------------------------------------
+// Host prologue (prologue.cpp)
+// ---------------------------------------------------------------------------
+
+struct HostScene {
+    std::vector<float> vert32;  // 3 floats per mesh vertex (ENU, FP32)
+    std::vector<int> tri;       // 3 vertex indices per triangle
+    std::vector<CellFrame> frame;
+    long long num_mesh_vertices = 0;
+    double lon0 = 0.0, lat0 = 0.0;
+};
+
+int prologue(const double* clon, const double* clat, const double* hsurf, const double* vlon,
+             const double* vlat, const int* cells_of_vertex, int num_cell, int num_vertex,
+             int grid_type, double ray_org_elev, HostScene& out, std::string& err);
+
+WalkParams make_walk_params(int num_hori, int refine_factor, int svf_type, double dist_search,
+                            int ncell);
+
+// ---------------------------------------------------------------------------
+// LBVH builder (lbvh.hip)
+// ---------------------------------------------------------------------------
+
+struct LbvhWork {
+    int capacity = 0;  // triangles the buffers hold
+    uint64_t* keys[2] = {nullptr, nullptr};
+    int* vals[2] = {nullptr, nullptr};
+    float4* aabb_lo = nullptr;  // leaves [0, n), interior [n, 2n-1)
+    float4* aabb_hi = nullptr;
+    int* child = nullptr;   // 2 per interior node
+    int* range = nullptr;   // 2 per interior node (first, last leaf)
+    int* parent = nullptr;  // leaves [0, n), interior [n, 2n-1)
+    int* flags = nullptr;   // refit arrival counters, per interior node
+    unsigned* bounds = nullptr;  // 6 order-preserving encoded floats + depth + spare
+    void* sort_tmp = nullptr;
+    size_t sort_tmp_bytes = 0;
+};
+
+int lbvh_reserve(LbvhWork& w, int n, std::string& err);
+void lbvh_free(LbvhWork& w);
+// Builds the BVH of ntri triangles (vertex indices into vert, 3 floats per
+// vertex).  out.nodes / out.tris must hold max(1, ntri - 1) nodes and 3 * ntri
+// float4.  Enqueued on stream; depth is read back only when want_depth.
+int lbvh_build(LbvhWork& w, const float* vert, const int* tri, int ntri, int leaf_size,
+               hipStream_t stream, DeviceBvh& out, bool want_depth, std::string& err);
+
+// Morton order of the cell ray origins (2D, x/y of the ENU frame).
+int order_cells(LbvhWork& w, const CellFrame* cells, int ncell, int* order, hipStream_t stream,
+                std::string& err);
+int order_reserve(LbvhWork& w, int ncell, std::string& err);
+
+// ---------------------------------------------------------------------------
+// Trace kernels (trace.hip)
+// ---------------------------------------------------------------------------
+
+struct TraceCounters {
+    unsigned long long rays;
+    unsigned long long nodes;
+    unsigned long long tris;
+    unsigned long long error;
+};
+
+int launch_trace(const DeviceBvh& bvh, const CellFrame* cells, const int* order,
+                 const WalkParams& p, int mode, int block_size, bool counters, double* horizon,
+                 double* skyview, TraceCounters* dcount, hipStream_t stream, std::string& err);
+
+}  // namespace hz
+
+#define HZ_HIP(call, errstr)                                                            \
+    do {                                                                                \
+        hipError_t e_ = (call);                                                         \
+        if (e_ != hipSuccess) {                                                         \
+            errstr = std::string(#call) + ": " + hipGetErrorString(e_);                 \
+            return -2;                                                                  \
+        }                                                                               \
+    } while (0)

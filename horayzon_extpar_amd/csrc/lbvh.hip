@@ -1,0 +1,458 @@
+// On-device LBVH builder (replaces the reference's Embree rtcCommitScene,
+// mo_lradtopo_horayzon.cpp:395-432).
+//
+//   1. k_tri_bounds   triangle boxes and the centroid bounds of the scene
+//   2. k_morton       63-bit Morton codes of the centroids in a cube of the
+//                     largest extent (a heightfield then splits mostly in x/y)
+//   3. rocprim radix sort of (code, triangle) pairs
+//   4. k_karras       Karras 2012 hierarchy ("Maximizing parallelism in the
+//                     construction of BVHs, octrees and k-d trees", HPG 2012),
+//                     duplicate codes broken by leaf index
+//   5. k_refit        bottom-up boxes, one arrival counter per interior node
+//   6. k_pack         64-byte traversal nodes holding both children's
+//                     conservative boxes; subtrees of <= leaf_size triangles
+//                     collapse into one leaf; triangles written in leaf order
+//
+// Every box is grown by 1 m + 1e-5 |coordinate| so that the FP32 slab test
+// can never reject a ray that the watertight triangle test accepts: the
+// occlusion answer is then the OR of the triangle test over all triangles,
+// whatever the tree looks like.
+
+#include <cstring>
+
+#include <rocprim/rocprim.hpp>
+
+#include "hz_internal.h"
+
+namespace hz {
+namespace {
+
+constexpr int kBlock = 256;
+
+__device__ __forceinline__ unsigned f2ord(float f) {
+    unsigned u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float ord2f(unsigned u) {
+    return __uint_as_float((u & 0x80000000u) ? (u & 0x7fffffffu) : ~u);
+}
+
+__global__ void k_init_bounds(unsigned* b) {
+    int t = threadIdx.x;
+    if (t < 3) b[t] = 0xffffffffu;        // min x, y, z
+    else if (t < 6) b[t] = 0u;            // max x, y, z
+    else if (t < 8) b[t] = 0u;            // depth, spare
+}
+
+__global__ __launch_bounds__(kBlock) void k_tri_bounds(const float* __restrict__ vert,
+                                                      const int* __restrict__ tri, int n,
+                                                      float4* __restrict__ lo,
+                                                      float4* __restrict__ hi,
+                                                      unsigned* __restrict__ bounds) {
+    int i = blockIdx.x * kBlock + threadIdx.x;
+    float cmin[3] = {3.0e38f, 3.0e38f, 3.0e38f}, cmax[3] = {-3.0e38f, -3.0e38f, -3.0e38f};
+    if (i < n) {
+        int a = tri[3 * i], b = tri[3 * i + 1], c = tri[3 * i + 2];
+        float l[3], h[3];
+        for (int k = 0; k < 3; k++) {
+            float p0 = vert[3 * a + k], p1 = vert[3 * b + k], p2 = vert[3 * c + k];
+            l[k] = fminf(p0, fminf(p1, p2));
+            h[k] = fmaxf(p0, fmaxf(p1, p2));
+            float cc = 0.5f * (l[k] + h[k]);
+            cmin[k] = cc;
+            cmax[k] = cc;
+        }
+        lo[i] = make_float4(l[0], l[1], l[2], 0.0f);
+        hi[i] = make_float4(h[0], h[1], h[2], 0.0f);
+    }
+    // wave reduction then one atomic per wave
+    for (int k = 0; k < 3; k++) {
+        for (int off = 32; off > 0; off >>= 1) {
+            cmin[k] = fminf(cmin[k], __shfl_xor(cmin[k], off, 64));
+            cmax[k] = fmaxf(cmax[k], __shfl_xor(cmax[k], off, 64));
+        }
+    }
+    if ((threadIdx.x & 63) == 0) {
+        for (int k = 0; k < 3; k++) {
+            if (cmin[k] <= cmax[k]) {
+                atomicMin(&bounds[k], f2ord(cmin[k]));
+                atomicMax(&bounds[3 + k], f2ord(cmax[k]));
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ uint64_t spread21(uint64_t x) {
+    x &= 0x1fffffull;
+    x = (x | (x << 32)) & 0x1f00000000ffffull;
+    x = (x | (x << 16)) & 0x1f0000ff0000ffull;
+    x = (x | (x << 8)) & 0x100f00f00f00f00full;
+    x = (x | (x << 4)) & 0x10c30c30c30c30c3ull;
+    x = (x | (x << 2)) & 0x1249249249249249ull;
+    return x;
+}
+
+__global__ __launch_bounds__(kBlock) void k_morton(const float4* __restrict__ lo,
+                                                  const float4* __restrict__ hi, int n,
+                                                  const unsigned* __restrict__ bounds,
+                                                  uint64_t* __restrict__ keys,
+                                                  int* __restrict__ vals) {
+    int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    float bmin[3], ext = 0.0f;
+    for (int k = 0; k < 3; k++) {
+        bmin[k] = ord2f(bounds[k]);
+        ext = fmaxf(ext, ord2f(bounds[3 + k]) - bmin[k]);
+    }
+    float inv = ext > 0.0f ? 1.0f / ext : 0.0f;
+    float4 l = lo[i], h = hi[i];
+    float c[3] = {0.5f * (l.x + h.x), 0.5f * (l.y + h.y), 0.5f * (l.z + h.z)};
+    uint64_t code = 0;
+    for (int k = 0; k < 3; k++) {
+        float q = (c[k] - bmin[k]) * inv;
+        q = fminf(fmaxf(q, 0.0f), 1.0f);
+        uint64_t v = (uint64_t)fminf(q * 2097152.0f, 2097151.0f);
+        code |= spread21(v) << (2 - k);
+    }
+    keys[i] = code;
+    vals[i] = i;
+}
+
+__device__ __forceinline__ int delta(const uint64_t* __restrict__ k, int n, int i, int j) {
+    if (j < 0 || j >= n) return -1;
+    uint64_t a = k[i], b = k[j];
+    if (a == b) return 64 + __clz((unsigned)(i ^ j));
+    return __clzll(a ^ b);
+}
+
+// Interior node i of n-1; children encoded: >= 0 interior, < 0 leaf ~index.
+__global__ __launch_bounds__(kBlock) void k_karras(const uint64_t* __restrict__ keys, int n,
+                                                  int* __restrict__ child,
+                                                  int* __restrict__ range,
+                                                  int* __restrict__ parent) {
+    int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n - 1) return;
+    int d = (delta(keys, n, i, i + 1) - delta(keys, n, i, i - 1)) >= 0 ? 1 : -1;
+    int dmin = delta(keys, n, i, i - d);
+    int lmax = 2;
+    while (delta(keys, n, i, i + lmax * d) > dmin) lmax <<= 1;
+    int l = 0;
+    for (int t = lmax >> 1; t >= 1; t >>= 1)
+        if (delta(keys, n, i, i + (l + t) * d) > dmin) l += t;
+    int j = i + l * d;
+    int dnode = delta(keys, n, i, j);
+    int s = 0;
+    int div = 2;
+    int t = (l + div - 1) / div;
+    while (true) {
+        if (delta(keys, n, i, i + (s + t) * d) > dnode) s += t;
+        if (t == 1) break;
+        div <<= 1;
+        t = (l + div - 1) / div;
+    }
+    int g = i + s * d + (d < 0 ? -1 : 0);
+    int first = min(i, j), last = max(i, j);
+    int left = (first == g) ? ~g : g;
+    int right = (last == g + 1) ? ~(g + 1) : g + 1;
+    child[2 * i] = left;
+    child[2 * i + 1] = right;
+    range[2 * i] = first;
+    range[2 * i + 1] = last;
+    // parent slots: leaves [0, n), interior [n, 2n - 1)
+    parent[left < 0 ? ~left : n + left] = i;
+    parent[right < 0 ? ~right : n + right] = i;
+    if (i == 0) parent[n] = -1;
+}
+
+__global__ __launch_bounds__(kBlock) void k_zero(int* __restrict__ p, int n) {
+    int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+// Leaf boxes were written in original triangle order; the refit reads them
+// through the sorted permutation.
+__global__ __launch_bounds__(kBlock) void k_refit(const int* __restrict__ sorted, int n,
+                                                 float4* __restrict__ lo, float4* __restrict__ hi,
+                                                 const int* __restrict__ child,
+                                                 const int* __restrict__ parent,
+                                                 int* __restrict__ flags) {
+    int leaf = blockIdx.x * kBlock + threadIdx.x;
+    if (leaf >= n) return;
+    int p = parent[leaf];
+    // a valid tree has < 128 levels (63 code bits + 32 index bits); the cap
+    // only bounds the loop
+    for (int guard = 0; p >= 0 && guard < 256; guard++) {
+        __threadfence();
+        if (atomicAdd(&flags[p], 1) == 0) return;  // first arrival: sibling not ready
+        __threadfence();
+        float4 l[2], h[2];
+        for (int s = 0; s < 2; s++) {
+            int c = child[2 * p + s];
+            int slot = c < 0 ? sorted[~c] : n + c;
+            // the agent-scope fence above orders this read after the
+            // sibling's write (which preceded its own fence and atomic)
+            l[s] = lo[slot];
+            h[s] = hi[slot];
+        }
+        lo[n + p] = make_float4(fminf(l[0].x, l[1].x), fminf(l[0].y, l[1].y),
+                                fminf(l[0].z, l[1].z), 0.0f);
+        hi[n + p] = make_float4(fmaxf(h[0].x, h[1].x), fmaxf(h[0].y, h[1].y),
+                                fmaxf(h[0].z, h[1].z), 0.0f);
+        p = parent[n + p];
+    }
+}
+
+__device__ __forceinline__ float grow_lo(float v) { return v - (1.0f + 1e-5f * fabsf(v)); }
+__device__ __forceinline__ float grow_hi(float v) { return v + (1.0f + 1e-5f * fabsf(v)); }
+
+__global__ __launch_bounds__(kBlock) void k_pack(const int* __restrict__ sorted, int n,
+                                                const float4* __restrict__ lo,
+                                                const float4* __restrict__ hi,
+                                                const int* __restrict__ child,
+                                                const int* __restrict__ range, int leaf_size,
+                                                BvhNode* __restrict__ nodes) {
+    int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n - 1) return;
+    float4 l[2], h[2];
+    int ref[2];
+    for (int s = 0; s < 2; s++) {
+        int c = child[2 * i + s];
+        if (c < 0) {
+            int leaf = ~c;
+            int slot = sorted[leaf];
+            l[s] = lo[slot];
+            h[s] = hi[slot];
+            ref[s] = ~leaf;  // count 1
+        } else {
+            l[s] = lo[n + c];
+            h[s] = hi[n + c];
+            int first = range[2 * c], last = range[2 * c + 1];
+            int cnt = last - first + 1;
+            ref[s] = (cnt <= leaf_size) ? ~(first | ((cnt - 1) << kLeafFirstBits)) : c;
+        }
+    }
+    BvhNode nd;
+    nd.lx0 = grow_lo(l[0].x);
+    nd.lx1 = grow_lo(l[1].x);
+    nd.hx0 = grow_hi(h[0].x);
+    nd.hx1 = grow_hi(h[1].x);
+    nd.ly0 = grow_lo(l[0].y);
+    nd.ly1 = grow_lo(l[1].y);
+    nd.hy0 = grow_hi(h[0].y);
+    nd.hy1 = grow_hi(h[1].y);
+    nd.lz0 = grow_lo(l[0].z);
+    nd.lz1 = grow_lo(l[1].z);
+    nd.hz0 = grow_hi(h[0].z);
+    nd.hz1 = grow_hi(h[1].z);
+    nd.r0 = ref[0];
+    nd.r1 = ref[1];
+    nd.pad0 = 0;
+    nd.pad1 = 0;
+    nodes[i] = nd;
+}
+
+// One triangle: both children of the root are the same leaf.
+__global__ void k_pack_single(const float4* __restrict__ lo, const float4* __restrict__ hi,
+                              BvhNode* __restrict__ nodes) {
+    if (threadIdx.x != 0) return;
+    float4 l = lo[0], h = hi[0];
+    BvhNode nd;
+    nd.lx0 = nd.lx1 = grow_lo(l.x);
+    nd.ly0 = nd.ly1 = grow_lo(l.y);
+    nd.lz0 = nd.lz1 = grow_lo(l.z);
+    nd.hx0 = nd.hx1 = grow_hi(h.x);
+    nd.hy0 = nd.hy1 = grow_hi(h.y);
+    nd.hz0 = nd.hz1 = grow_hi(h.z);
+    nd.r0 = nd.r1 = ~0;
+    nd.pad0 = nd.pad1 = 0;
+    nodes[0] = nd;
+}
+
+__global__ __launch_bounds__(kBlock) void k_tris(const int* __restrict__ sorted, int n,
+                                                const float* __restrict__ vert,
+                                                const int* __restrict__ tri,
+                                                float4* __restrict__ out) {
+    int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= n) return;
+    int t = sorted[k];
+    int a = tri[3 * t], b = tri[3 * t + 1], c = tri[3 * t + 2];
+    out[3 * k] = make_float4(vert[3 * a], vert[3 * a + 1], vert[3 * a + 2], vert[3 * b]);
+    out[3 * k + 1] = make_float4(vert[3 * b + 1], vert[3 * b + 2], vert[3 * c], vert[3 * c + 1]);
+    out[3 * k + 2] = make_float4(vert[3 * c + 2], __int_as_float(t), 0.0f, 0.0f);
+}
+
+// Depth of every interior node's deepest leaf path: each leaf walks to the
+// root counting levels (diagnostics and stack sizing).
+__global__ __launch_bounds__(kBlock) void k_depth(const int* __restrict__ parent, int n,
+                                                 unsigned* __restrict__ out) {
+    int leaf = blockIdx.x * kBlock + threadIdx.x;
+    if (leaf >= n) return;
+    int p = parent[leaf];
+    unsigned d = 0;
+    while (p >= 0 && d < 256) {
+        d++;
+        p = parent[n + p];
+    }
+    // d = number of interior ancestors; deepest interior level = d - 1
+    for (int off = 32; off > 0; off >>= 1) d = max(d, (unsigned)__shfl_xor((int)d, off, 64));
+    if ((threadIdx.x & 63) == 0) atomicMax(out, d);
+}
+
+// --- cell ordering ---------------------------------------------------------
+
+__global__ __launch_bounds__(kBlock) void k_cell_bounds(const CellFrame* __restrict__ cells,
+                                                       int n, unsigned* __restrict__ bounds) {
+    int i = blockIdx.x * kBlock + threadIdx.x;
+    float mn[2] = {3.0e38f, 3.0e38f}, mx[2] = {-3.0e38f, -3.0e38f};
+    if (i < n) {
+        for (int k = 0; k < 2; k++) mn[k] = mx[k] = cells[i].org[k];
+    }
+    for (int k = 0; k < 2; k++)
+        for (int off = 32; off > 0; off >>= 1) {
+            mn[k] = fminf(mn[k], __shfl_xor(mn[k], off, 64));
+            mx[k] = fmaxf(mx[k], __shfl_xor(mx[k], off, 64));
+        }
+    if ((threadIdx.x & 63) == 0 && mn[0] <= mx[0]) {
+        for (int k = 0; k < 2; k++) {
+            atomicMin(&bounds[k], f2ord(mn[k]));
+            atomicMax(&bounds[3 + k], f2ord(mx[k]));
+        }
+    }
+}
+
+__device__ __forceinline__ uint64_t spread32(uint64_t x) {
+    x &= 0xffffffffull;
+    x = (x | (x << 16)) & 0x0000ffff0000ffffull;
+    x = (x | (x << 8)) & 0x00ff00ff00ff00ffull;
+    x = (x | (x << 4)) & 0x0f0f0f0f0f0f0f0full;
+    x = (x | (x << 2)) & 0x3333333333333333ull;
+    x = (x | (x << 1)) & 0x5555555555555555ull;
+    return x;
+}
+
+__global__ __launch_bounds__(kBlock) void k_cell_morton(const CellFrame* __restrict__ cells,
+                                                       int n, const unsigned* __restrict__ bounds,
+                                                       uint64_t* __restrict__ keys,
+                                                       int* __restrict__ vals) {
+    int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    float bx = ord2f(bounds[0]), by = ord2f(bounds[1]);
+    float ext = fmaxf(ord2f(bounds[3]) - bx, ord2f(bounds[4]) - by);
+    float inv = ext > 0.0f ? 1.0f / ext : 0.0f;
+    float qx = fminf(fmaxf((cells[i].org[0] - bx) * inv, 0.0f), 1.0f);
+    float qy = fminf(fmaxf((cells[i].org[1] - by) * inv, 0.0f), 1.0f);
+    uint64_t ux = (uint64_t)fminf(qx * 16777216.0f, 16777215.0f);
+    uint64_t uy = (uint64_t)fminf(qy * 16777216.0f, 16777215.0f);
+    keys[i] = (spread32(uy) << 1) | spread32(ux);
+    vals[i] = i;
+}
+
+inline int grid(int n) { return (n + kBlock - 1) / kBlock; }
+
+}  // namespace
+
+int lbvh_reserve(LbvhWork& w, int n, std::string& err) {
+    if (n <= w.capacity) return 0;
+    lbvh_free(w);
+    int cap = n;
+    size_t nn = (size_t)cap;
+    HZ_HIP(hipMalloc(&w.keys[0], nn * 8), err);
+    HZ_HIP(hipMalloc(&w.keys[1], nn * 8), err);
+    HZ_HIP(hipMalloc(&w.vals[0], nn * 4), err);
+    HZ_HIP(hipMalloc(&w.vals[1], nn * 4), err);
+    HZ_HIP(hipMalloc(&w.aabb_lo, nn * 2 * sizeof(float4)), err);
+    HZ_HIP(hipMalloc(&w.aabb_hi, nn * 2 * sizeof(float4)), err);
+    HZ_HIP(hipMalloc(&w.child, nn * 2 * 4), err);
+    HZ_HIP(hipMalloc(&w.range, nn * 2 * 4), err);
+    HZ_HIP(hipMalloc(&w.parent, nn * 2 * 4), err);
+    HZ_HIP(hipMalloc(&w.flags, nn * 4), err);
+    HZ_HIP(hipMalloc(&w.bounds, 16 * 4), err);
+    size_t tmp = 0;
+    HZ_HIP(rocprim::radix_sort_pairs(nullptr, tmp, w.keys[0], w.keys[1], w.vals[0], w.vals[1],
+                                     (unsigned)cap, 0, 64),
+           err);
+    w.sort_tmp_bytes = tmp;
+    HZ_HIP(hipMalloc(&w.sort_tmp, tmp), err);
+    w.capacity = cap;
+    return 0;
+}
+
+void lbvh_free(LbvhWork& w) {
+    void* p[] = {w.keys[0], w.keys[1], w.vals[0], w.vals[1], w.aabb_lo, w.aabb_hi, w.child,
+                 w.range, w.parent, w.flags, w.bounds, w.sort_tmp};
+    for (void* q : p)
+        if (q) (void)hipFree(q);
+    w = LbvhWork();
+}
+
+int lbvh_build(LbvhWork& w, const float* vert, const int* tri, int n, int leaf_size,
+               hipStream_t st, DeviceBvh& out, bool want_depth, std::string& err) {
+    out.num_tri = n;
+    out.num_nodes = 0;
+    out.depth = 0;
+    if (n == 0) return 0;
+    if (lbvh_reserve(w, n, err)) return -3;
+    if (leaf_size < 1) leaf_size = 1;
+    if (leaf_size > kMaxLeafSize) leaf_size = kMaxLeafSize;
+    hipLaunchKernelGGL(k_init_bounds, dim3(1), dim3(64), 0, st, w.bounds);
+    hipLaunchKernelGGL(k_tri_bounds, dim3(grid(n)), dim3(kBlock), 0, st, vert, tri, n, w.aabb_lo,
+                       w.aabb_hi, w.bounds);
+    if (n == 1) {
+        hipLaunchKernelGGL(k_pack_single, dim3(1), dim3(64), 0, st, w.aabb_lo, w.aabb_hi,
+                           out.nodes);
+        hipLaunchKernelGGL(k_morton, dim3(1), dim3(kBlock), 0, st, w.aabb_lo, w.aabb_hi, n,
+                           w.bounds, w.keys[0], w.vals[0]);
+        hipLaunchKernelGGL(k_tris, dim3(1), dim3(kBlock), 0, st, w.vals[0], n, vert, tri,
+                           out.tris);
+        out.num_nodes = 1;
+        HZ_HIP(hipGetLastError(), err);
+        return 0;
+    }
+    hipLaunchKernelGGL(k_morton, dim3(grid(n)), dim3(kBlock), 0, st, w.aabb_lo, w.aabb_hi, n,
+                       w.bounds, w.keys[0], w.vals[0]);
+    size_t tmp = w.sort_tmp_bytes;
+    HZ_HIP(rocprim::radix_sort_pairs(w.sort_tmp, tmp, w.keys[0], w.keys[1], w.vals[0], w.vals[1],
+                                     (unsigned)n, 0, 63, st),
+           err);
+    const uint64_t* keys = w.keys[1];
+    const int* sorted = w.vals[1];
+    hipLaunchKernelGGL(k_karras, dim3(grid(n - 1)), dim3(kBlock), 0, st, keys, n, w.child,
+                       w.range, w.parent);
+    hipLaunchKernelGGL(k_zero, dim3(grid(n - 1)), dim3(kBlock), 0, st, w.flags, n - 1);
+    hipLaunchKernelGGL(k_refit, dim3(grid(n)), dim3(kBlock), 0, st, sorted, n, w.aabb_lo,
+                       w.aabb_hi, w.child, w.parent, w.flags);
+    hipLaunchKernelGGL(k_pack, dim3(grid(n - 1)), dim3(kBlock), 0, st, sorted, n, w.aabb_lo,
+                       w.aabb_hi, w.child, w.range, leaf_size, out.nodes);
+    hipLaunchKernelGGL(k_tris, dim3(grid(n)), dim3(kBlock), 0, st, sorted, n, vert, tri,
+                       out.tris);
+    out.num_nodes = n - 1;
+    if (want_depth) {
+        hipLaunchKernelGGL(k_depth, dim3(grid(n)), dim3(kBlock), 0, st, w.parent, n,
+                           w.bounds + 6);
+        unsigned d = 0;
+        HZ_HIP(hipMemcpyAsync(&d, w.bounds + 6, 4, hipMemcpyDeviceToHost, st), err);
+        HZ_HIP(hipStreamSynchronize(st), err);
+        out.depth = d > 0 ? (int)d - 1 : 0;
+    }
+    HZ_HIP(hipGetLastError(), err);
+    return 0;
+}
+
+int order_reserve(LbvhWork& w, int ncell, std::string& err) { return lbvh_reserve(w, ncell, err); }
+
+int order_cells(LbvhWork& w, const CellFrame* cells, int n, int* order, hipStream_t st,
+                std::string& err) {
+    if (lbvh_reserve(w, n, err)) return -3;
+    hipLaunchKernelGGL(k_init_bounds, dim3(1), dim3(64), 0, st, w.bounds);
+    hipLaunchKernelGGL(k_cell_bounds, dim3(grid(n)), dim3(kBlock), 0, st, cells, n, w.bounds);
+    hipLaunchKernelGGL(k_cell_morton, dim3(grid(n)), dim3(kBlock), 0, st, cells, n, w.bounds,
+                       w.keys[0], w.vals[0]);
+    size_t tmp = w.sort_tmp_bytes;
+    HZ_HIP(rocprim::radix_sort_pairs(w.sort_tmp, tmp, w.keys[0], w.keys[1], w.vals[0], order,
+                                     (unsigned)n, 0, 48, st),
+           err);
+    HZ_HIP(hipGetLastError(), err);
+    return 0;
+}
+
+}  // namespace hz

@@ -1,0 +1,270 @@
+// Host FP64 prologue: ICON cells/vertices -> triangle mesh -> spherical
+// ECEF -> local ENU -> FP32 scene and per-cell ray frames.
+//
+// This is the only FP64 geometry of the library.  It follows
+//   build_triangle_mesh      mo_lradtopo_horayzon.cpp:228-317
+//   lonlat2ecef              :158-169
+//   ENU origin + ecef2enu    :178-201, :610-644
+//   per-cell frame / origin  :693-722
+// with the same arithmetic (operation order included), so that the FP32
+// vertices and ray origins handed to the device are the values the reference
+// hands to Embree (:406-410, :715-720).  Work is split over host threads; the
+// only order-dependent reduction (the ENU origin sum, :627-631) stays
+// sequential in cell order.
+
+#include <algorithm>
+#include <cmath>
+#include <thread>
+
+#include "hz_internal.h"
+
+namespace hz {
+namespace {
+
+constexpr double kPi = 3.14159265358979323846;
+constexpr double kRadEarth = 6371229.0;  // reference :584
+
+inline double deg2rad(double a) { return (a / 180.0) * kPi; }
+
+// sin and cos of one argument through glibc's sincos: the reference is built
+// with gcc (setup.py:18-19), which turns its paired sin(x) / cos(x) calls
+// into one sincos(x) call, and glibc's sincos does not always return the
+// bits of separate sin and cos calls.
+inline void sin_cos(double x, double* s, double* c) { ::sincos(x, s, c); }
+
+template <class F>
+void parallel_for(long long n, F&& f) {
+    unsigned hw = std::thread::hardware_concurrency();
+    int nt = (int)std::min<unsigned>(hw ? hw : 1, 16);
+    if (n < 20000) nt = 1;
+    if (nt <= 1) {
+        f(0LL, n);
+        return;
+    }
+    std::vector<std::thread> th;
+    long long chunk = (n + nt - 1) / nt;
+    for (int t = 0; t < nt; t++) {
+        long long b = t * chunk, e = std::min(n, b + chunk);
+        if (b >= e) break;
+        th.emplace_back([&f, b, e]() { f(b, e); });
+    }
+    for (auto& x : th) x.join();
+}
+
+// Valid adjacent cells of ICON vertex v, ordered clockwise from the
+// latitude axis (reference :243-261; for <= 6 keys std::sort keeps equal
+// angles in input order, which this insertion sort reproduces).
+inline int sorted_cells(const double* clon, const double* clat, const double* vlon,
+                        const double* vlat, const int* cov, int nvert, int v, int* cell,
+                        double* hsum, const double* hsurf) {
+    double ang[6];
+    int c6[6];
+    int n = 0;
+    double hs = 0.0;
+    for (int j = 0; j < 6; j++) {
+        int c = cov[(size_t)nvert * j + v];
+        if (c == -2) continue;
+        double a = std::atan2(clon[c] - vlon[v], clat[c] - vlat[v]);
+        if (a < 0.0) a += 2.0 * kPi;
+        ang[n] = a;
+        c6[n] = c;
+        hs += hsurf[c];
+        n++;
+    }
+    int idx[6];
+    for (int i = 0; i < n; i++) idx[i] = i;
+    for (int i = 1; i < n; i++) {
+        int t = idx[i];
+        int j = i - 1;
+        while (j >= 0 && ang[t] < ang[idx[j]]) {
+            idx[j + 1] = idx[j];
+            j--;
+        }
+        idx[j + 1] = t;
+    }
+    for (int i = 0; i < n; i++) cell[i] = c6[idx[i]];
+    *hsum = hs;
+    return n;
+}
+
+}  // namespace
+
+WalkParams make_walk_params(int num_hori, int refine_factor, int svf_type, double dist_search,
+                            int ncell) {
+    WalkParams p;
+    int L = num_hori * refine_factor;
+    p.hori_acc = deg2rad(0.25);
+    p.elev_thresh = deg2rad(-85.0);
+    sin_cos(deg2rad(360.0) / (double)L, &p.azim_sin, &p.azim_cos);
+    sin_cos(2.0 * p.hori_acc, &p.elev_sin_2ha, &p.elev_cos_2ha);
+    double shift = 0.0;
+    if (refine_factor != 1)
+        shift = -(deg2rad(360.0) / (2.0 * num_hori)) + (deg2rad(360.0) / (2.0 * (double)L));
+    sin_cos(-shift, &p.shift_sin, &p.shift_cos);
+    p.tfar = (float)dist_search;
+    p.num_hori = num_hori;
+    p.refine_factor = refine_factor;
+    p.svf_type = svf_type;
+    p.ncell = ncell;
+    return p;
+}
+
+int prologue(const double* clon, const double* clat, const double* hsurf, const double* vlon,
+             const double* vlat, const int* cov, int ncell, int nvert, int grid_type,
+             double ray_org_elev, HostScene& out, std::string& err) {
+    if (ncell < 1 || nvert < 0) {
+        err = "num_cell must be >= 1 and num_vertex >= 0";
+        return -1;
+    }
+    // --- pass 1: triangles (and added vertices) per ICON vertex -------------
+    std::vector<int> ntri_v(nvert), nadd_v(nvert);
+    parallel_for(nvert, [&](long long b, long long e) {
+        for (long long v = b; v < e; v++) {
+            int n = 0;
+            for (int j = 0; j < 6; j++)
+                if (cov[(size_t)nvert * j + v] != -2) n++;
+            if (grid_type == 0) {
+                ntri_v[v] = n >= 3 ? n - 2 : 0;
+                nadd_v[v] = 0;
+            } else {
+                ntri_v[v] = n == 6 ? 6 : 0;
+                nadd_v[v] = n == 6 ? 1 : 0;
+            }
+        }
+    });
+    std::vector<long long> toff(nvert + 1), aoff(nvert + 1);
+    toff[0] = aoff[0] = 0;
+    for (int v = 0; v < nvert; v++) {
+        toff[v + 1] = toff[v] + ntri_v[v];
+        aoff[v + 1] = aoff[v] + nadd_v[v];
+    }
+    long long ntri = toff[nvert];
+    long long nmv = (long long)ncell + aoff[nvert];
+    if (ntri > (long long)kLeafFirstMask) {
+        err = "mesh has more triangles than the library supports (2^28)";
+        return -1;
+    }
+    // mesh vertices as lon, lat, elevation (reference :234-238, :300-301)
+    std::vector<double> P((size_t)nmv * 3);
+    out.tri.assign((size_t)ntri * 3, 0);
+    parallel_for(ncell, [&](long long b, long long e) {
+        for (long long i = b; i < e; i++) {
+            P[3 * i] = clon[i];
+            P[3 * i + 1] = clat[i];
+            P[3 * i + 2] = hsurf[i];
+        }
+    });
+    // --- pass 2: fill triangles (reference :243-312) -------------------------
+    parallel_for(nvert, [&](long long b, long long e) {
+        int cell[6];
+        double hs;
+        for (long long v = b; v < e; v++) {
+            if (ntri_v[v] == 0) continue;
+            int n = sorted_cells(clon, clat, vlon, vlat, cov, nvert, (int)v, cell, &hs, hsurf);
+            int* t = &out.tri[(size_t)toff[v] * 3];
+            if (grid_type == 0) {
+                for (int j = 1; j + 1 < n; j++) {  // fan about the first cell
+                    *t++ = cell[0];
+                    *t++ = cell[j];
+                    *t++ = cell[j + 1];
+                }
+            } else {
+                long long add = (long long)ncell + aoff[v];
+                P[3 * add] = vlon[v];
+                P[3 * add + 1] = vlat[v];
+                P[3 * add + 2] = hs / 6.0;
+                for (int j = 0; j < 6; j++) {
+                    *t++ = cell[j];
+                    *t++ = cell[(j + 1) % 6];
+                    *t++ = (int)add;
+                }
+            }
+        }
+    });
+    // --- spherical ECEF (reference :158-169) --------------------------------
+    const double R = kRadEarth;
+    parallel_for(nmv, [&](long long b, long long e) {
+        for (long long i = b; i < e; i++) {
+            double sl, cl, sp, cp;
+            sin_cos(P[3 * i], &sl, &cl);
+            sin_cos(P[3 * i + 1], &sp, &cp);
+            double h = P[3 * i + 2];
+            P[3 * i] = (R + h) * cp * cl;
+            P[3 * i + 1] = (R + h) * cp * sl;
+            P[3 * i + 2] = (R + h) * sp;
+        }
+    });
+    // --- ENU origin: centroid of the cell points, summed in cell order
+    //     (reference :624-634) ------------------------------------------------
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    for (int i = 0; i < ncell; i++) {
+        sx += P[3 * (size_t)i];
+        sy += P[3 * (size_t)i + 1];
+        sz += P[3 * (size_t)i + 2];
+    }
+    double rad = std::sqrt(sx * sx + sy * sy + sz * sz);
+    double lon0 = std::atan2(sy, sx);
+    double lat0 = std::asin(sz / rad);
+    out.lon0 = lon0;
+    out.lat0 = lat0;
+    // --- ECEF -> ENU (reference :178-201) ------------------------------------
+    double so, co, sa, ca;
+    sin_cos(lon0, &so, &co);
+    sin_cos(lat0, &sa, &ca);
+    const double x0 = R * ca * co;
+    const double y0 = R * ca * so;
+    const double z0 = R * sa;
+    auto enu = [&](double x, double y, double z, double* r) {
+        double dx = x - x0, dy = y - y0, dz = z - z0;
+        r[0] = -so * dx + co * dy;
+        r[1] = -sa * co * dx - sa * so * dy + ca * dz;
+        r[2] = ca * co * dx + ca * so * dy + sa * dz;
+    };
+    double centre[3], pole[3];
+    enu(0.0, 0.0, 0.0, centre);
+    enu(0.0, 0.0, R, pole);
+    out.vert32.resize((size_t)nmv * 3);
+    parallel_for(nmv, [&](long long b, long long e) {
+        for (long long i = b; i < e; i++) {
+            double r[3];
+            enu(P[3 * i], P[3 * i + 1], P[3 * i + 2], r);
+            P[3 * i] = r[0];
+            P[3 * i + 1] = r[1];
+            P[3 * i + 2] = r[2];
+            // FP32 scene vertex (reference :406-410)
+            out.vert32[3 * i] = (float)r[0];
+            out.vert32[3 * i + 1] = (float)r[1];
+            out.vert32[3 * i + 2] = (float)r[2];
+        }
+    });
+    // --- per-cell frame (reference :693-722) ---------------------------------
+    out.frame.resize(ncell);
+    parallel_for(ncell, [&](long long b, long long e) {
+        for (long long i = b; i < e; i++) {
+            const double* p = &P[3 * i];
+            CellFrame& f = out.frame[i];
+            double n[3] = {p[0] - centre[0], p[1] - centre[1], p[2] - centre[2]};
+            double m = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+            n[0] /= m;
+            n[1] /= m;
+            n[2] /= m;
+            double vn[3] = {pole[0] - p[0], pole[1] - p[1], pole[2] - p[2]};
+            double dp = vn[0] * n[0] + vn[1] * n[1] + vn[2] * n[2];
+            double N[3] = {vn[0] - dp * n[0], vn[1] - dp * n[1], vn[2] - dp * n[2]};
+            double mN = std::sqrt(N[0] * N[0] + N[1] * N[1] + N[2] * N[2]);
+            N[0] /= mN;
+            N[1] /= mN;
+            N[2] /= mN;
+            for (int a = 0; a < 3; a++) {
+                f.n[a] = n[a];
+                f.north[a] = N[a];
+                f.org[a] = (float)(p[a] + n[a] * ray_org_elev);
+            }
+            f.pad = 0;
+        }
+    });
+    out.num_mesh_vertices = nmv;
+    return 0;
+}
+
+}  // namespace hz

@@ -1,0 +1,393 @@
+// Horizon search kernel: one lane per cell, the whole elevation search of
+// that cell (all nhori * refine_factor azimuths) plus the clip, sector mean
+// and sky view factor fused behind it.  Replaces the TBB loop and the
+// per-ray Embree queries of the reference (mo_lradtopo_horayzon.cpp:686-761,
+// :490-555, :446-460).
+//
+// Numerics.  The walk state (ray direction, elevation angle, rotation axes)
+// is FP64 and is updated with the reference's Rodrigues rotations in the
+// reference's operation order (:119-130, :507-551); the file is compiled
+// with -ffp-contract=off, so the FP32 ray handed to the occlusion query is
+// the one the reference builds.  The occlusion query is the watertight
+// ray/triangle test of Woop, Benthin & Wald (JCGT 2013), two-sided, accepting
+// 0 < t <= (float)dist_search, over a BVH whose boxes are conservative
+// (see lbvh.hip): its answer equals the OR of the triangle test over every
+// triangle, independent of the tree.
+//
+// Layout.  Lanes of a wavefront take cells that are neighbours in Morton
+// order, so their rays start close together and share BVH nodes in the
+// caches.  Workgroups are remapped so that each XCD (workgroups are dealt
+// round-robin over the 8 XCDs) walks one contiguous stretch of that order,
+// keeping each L2's working set spatially compact.  The traversal stack
+// keeps its first kLdsStack entries in LDS (entry-major, so lanes hit
+// distinct banks) and spills deeper entries to a private array.
+
+#include "hz_internal.h"
+
+namespace hz {
+namespace {
+
+constexpr double kPi = 3.14159265358979323846;
+constexpr int kLdsStack = 32;
+constexpr int kStackMax = 128;  // > the LBVH depth bound (63 code bits + 32 index bits)
+
+struct D3 {
+    double x, y, z;
+};
+
+__device__ __forceinline__ D3 rodrigues(const D3& v, const D3& k, double s, double c) {
+    double term = (k.x * v.x + k.y * v.y + k.z * v.z) * (1.0 - c);
+    D3 r;
+    r.x = v.x * c + (k.y * v.z - k.z * v.y) * s + k.x * term;
+    r.y = v.y * c + (k.z * v.x - k.x * v.z) * s + k.y * term;
+    r.z = v.z * c + (k.x * v.y - k.y * v.x) * s + k.z * term;
+    return r;
+}
+
+__device__ __forceinline__ D3 unit_cross(const D3& a, const D3& b) {
+    D3 c = {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+    double m = sqrt(c.x * c.x + c.y * c.y + c.z * c.z);
+    c.x /= m;
+    c.y /= m;
+    c.z /= m;
+    return c;
+}
+
+__device__ __forceinline__ float sel3(float a, float b, float c, int k) {
+    return k == 0 ? a : (k == 1 ? b : c);
+}
+
+// Per-ray constants of the watertight test and of the slab test.
+struct Ray {
+    float okx, oky, okz;  // origin, permuted
+    float o0, o1, o2;
+    float Sx, Sy, Sz;
+    int kx, ky, kz;
+    float ix, iy, iz;     // 1 / direction (zero components replaced by +-1e-20)
+    float oix, oiy, oiz;  // origin * (1 / direction)
+    float tfar, tfar_box;
+};
+
+__device__ __forceinline__ Ray make_ray(float o0, float o1, float o2, float d0, float d1,
+                                        float d2, float tfar) {
+    Ray r;
+    float a0 = fabsf(d0), a1 = fabsf(d1), a2 = fabsf(d2);
+    int kz = a1 > a0 ? 1 : 0;
+    float akz = kz ? a1 : a0;
+    if (a2 > akz) kz = 2;
+    int kx = kz == 2 ? 0 : kz + 1;
+    int ky = kx == 2 ? 0 : kx + 1;
+    float dkz = sel3(d0, d1, d2, kz);
+    if (dkz < 0.0f) {
+        int t = kx;
+        kx = ky;
+        ky = t;
+    }
+    float dkx = sel3(d0, d1, d2, kx), dky = sel3(d0, d1, d2, ky);
+    r.Sx = dkx / dkz;
+    r.Sy = dky / dkz;
+    r.Sz = 1.0f / dkz;
+    r.kx = kx;
+    r.ky = ky;
+    r.kz = kz;
+    r.o0 = o0;
+    r.o1 = o1;
+    r.o2 = o2;
+    r.okx = sel3(o0, o1, o2, kx);
+    r.oky = sel3(o0, o1, o2, ky);
+    r.okz = sel3(o0, o1, o2, kz);
+    float e0 = fabsf(d0) < 1e-20f ? copysignf(1e-20f, d0) : d0;
+    float e1 = fabsf(d1) < 1e-20f ? copysignf(1e-20f, d1) : d1;
+    float e2 = fabsf(d2) < 1e-20f ? copysignf(1e-20f, d2) : d2;
+    r.ix = 1.0f / e0;
+    r.iy = 1.0f / e1;
+    r.iz = 1.0f / e2;
+    r.oix = o0 * r.ix;
+    r.oiy = o1 * r.iy;
+    r.oiz = o2 * r.iz;
+    r.tfar = tfar;
+    r.tfar_box = tfar * 1.0001f + 1.0f;
+    return r;
+}
+
+// Watertight two-sided test; operation order fixed (no contraction).
+__device__ __forceinline__ bool tri_hit(const Ray& r, float4 t0, float4 t1, float4 t2) {
+    // vertices: v0 = t0.xyz, v1 = (t0.w, t1.x, t1.y), v2 = (t1.z, t1.w, t2.x)
+    float A0 = t0.x - r.o0, A1 = t0.y - r.o1, A2 = t0.z - r.o2;
+    float B0 = t0.w - r.o0, B1 = t1.x - r.o1, B2 = t1.y - r.o2;
+    float C0 = t1.z - r.o0, C1 = t1.w - r.o1, C2 = t2.x - r.o2;
+    float Akx = sel3(A0, A1, A2, r.kx), Aky = sel3(A0, A1, A2, r.ky), Akz = sel3(A0, A1, A2, r.kz);
+    float Bkx = sel3(B0, B1, B2, r.kx), Bky = sel3(B0, B1, B2, r.ky), Bkz = sel3(B0, B1, B2, r.kz);
+    float Ckx = sel3(C0, C1, C2, r.kx), Cky = sel3(C0, C1, C2, r.ky), Ckz = sel3(C0, C1, C2, r.kz);
+    float Ax = Akx - r.Sx * Akz;
+    float Ay = Aky - r.Sy * Akz;
+    float Bx = Bkx - r.Sx * Bkz;
+    float By = Bky - r.Sy * Bkz;
+    float Cx = Ckx - r.Sx * Ckz;
+    float Cy = Cky - r.Sy * Ckz;
+    float U = Cx * By - Cy * Bx;
+    float V = Ax * Cy - Ay * Cx;
+    float W = Bx * Ay - By * Ax;
+    if (U == 0.0f || V == 0.0f || W == 0.0f) {
+        U = (float)((double)Cx * (double)By - (double)Cy * (double)Bx);
+        V = (float)((double)Ax * (double)Cy - (double)Ay * (double)Cx);
+        W = (float)((double)Bx * (double)Ay - (double)By * (double)Ax);
+    }
+    if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
+    float det = U + V + W;
+    if (det == 0.0f) return false;
+    float Az = r.Sz * Akz;
+    float Bz = r.Sz * Bkz;
+    float Cz = r.Sz * Ckz;
+    float T = U * Az + V * Bz + W * Cz;
+    float Ts = det < 0.0f ? -T : T;
+    float ad = fabsf(det);
+    return (Ts > 0.0f) && (Ts <= r.tfar * ad);
+}
+
+template <bool COUNT>
+__device__ __forceinline__ bool leaf_hit(const Ray& r, const float4* __restrict__ tris, int ref,
+                                         unsigned& ntest) {
+    int code = ~ref;
+    int first = code & kLeafFirstMask;
+    int cnt = (code >> kLeafFirstBits) + 1;
+    for (int k = 0; k < cnt; k++) {
+        const float4* t = tris + 3 * (first + k);
+        float4 t0 = t[0], t1 = t[1], t2 = t[2];
+        if (COUNT) ntest++;
+        if (tri_hit(r, t0, t1, t2)) return true;
+    }
+    return false;
+}
+
+// Any-hit traversal; nearer child first.  Returns the occlusion answer.
+template <bool COUNT, int BLOCK>
+__device__ __forceinline__ bool occluded(const BvhNode* __restrict__ nodes,
+                                         const float4* __restrict__ tris, int* stk, const Ray& r,
+                                         unsigned& nvisit, unsigned& ntest, int& err) {
+    int ovf[kStackMax - kLdsStack];
+    int node = 0;
+    int sp = 0;
+    for (int guard = 0;; guard++) {
+        // a BVH of n triangles has n - 1 interior nodes; more fetches than
+        // that in one query can only come from a malformed tree
+        if (guard > (1 << 28)) {
+            err |= 4;
+            return true;
+        }
+        const float4* np = reinterpret_cast<const float4*>(nodes + node);
+        float4 a = np[0], b = np[1], c = np[2];
+        int4 q = reinterpret_cast<const int4*>(np)[3];
+        if (COUNT) nvisit++;
+        // child boxes: a = (lx0, lx1, hx0, hx1), b = (ly0, ly1, hy0, hy1), c = (lz0, lz1, hz0, hz1)
+        float x0l = __builtin_fmaf(a.x, r.ix, -r.oix), x0h = __builtin_fmaf(a.z, r.ix, -r.oix);
+        float x1l = __builtin_fmaf(a.y, r.ix, -r.oix), x1h = __builtin_fmaf(a.w, r.ix, -r.oix);
+        float y0l = __builtin_fmaf(b.x, r.iy, -r.oiy), y0h = __builtin_fmaf(b.z, r.iy, -r.oiy);
+        float y1l = __builtin_fmaf(b.y, r.iy, -r.oiy), y1h = __builtin_fmaf(b.w, r.iy, -r.oiy);
+        float z0l = __builtin_fmaf(c.x, r.iz, -r.oiz), z0h = __builtin_fmaf(c.z, r.iz, -r.oiz);
+        float z1l = __builtin_fmaf(c.y, r.iz, -r.oiz), z1h = __builtin_fmaf(c.w, r.iz, -r.oiz);
+        float tmin0 = fmaxf(fmaxf(fminf(x0l, x0h), fminf(y0l, y0h)), fmaxf(fminf(z0l, z0h), -1.0f));
+        float tmax0 = fminf(fminf(fmaxf(x0l, x0h), fmaxf(y0l, y0h)), fminf(fmaxf(z0l, z0h), r.tfar_box));
+        float tmin1 = fmaxf(fmaxf(fminf(x1l, x1h), fminf(y1l, y1h)), fmaxf(fminf(z1l, z1h), -1.0f));
+        float tmax1 = fminf(fminf(fmaxf(x1l, x1h), fmaxf(y1l, y1h)), fminf(fmaxf(z1l, z1h), r.tfar_box));
+        bool h0 = tmin0 <= tmax0;
+        bool h1 = tmin1 <= tmax1;
+        int r0 = q.x, r1 = q.y;
+        if (h0 && r0 < 0) {
+            if (leaf_hit<COUNT>(r, tris, r0, ntest)) return true;
+            h0 = false;
+        }
+        if (h1 && r1 < 0) {
+            if (leaf_hit<COUNT>(r, tris, r1, ntest)) return true;
+            h1 = false;
+        }
+        if (h0 && h1) {
+            bool first0 = tmin0 <= tmin1;
+            int nearc = first0 ? r0 : r1;
+            int farc = first0 ? r1 : r0;
+            if (sp >= kStackMax) {
+                err = 1;
+                return true;
+            }
+            if (sp < kLdsStack)
+                stk[sp * BLOCK] = farc;
+            else
+                ovf[sp - kLdsStack] = farc;
+            sp++;
+            node = nearc;
+        } else if (h0) {
+            node = r0;
+        } else if (h1) {
+            node = r1;
+        } else {
+            if (sp == 0) return false;
+            sp--;
+            node = sp < kLdsStack ? stk[sp * BLOCK] : ovf[sp - kLdsStack];
+        }
+    }
+}
+
+__device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+template <int MODE, bool COUNT, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_trace(const BvhNode* __restrict__ nodes,
+                                                 const float4* __restrict__ tris, int num_tri,
+                                                 const CellFrame* __restrict__ cells,
+                                                 const int* __restrict__ order, WalkParams P,
+                                                 int nblocks, double* __restrict__ horizon,
+                                                 double* __restrict__ skyview,
+                                                 TraceCounters* __restrict__ cnt) {
+    __shared__ int stk_all[kLdsStack * BLOCK];
+    // XCD-aware remap: gridDim.x is a multiple of 8; XCD x gets the
+    // contiguous block range [x * per, (x + 1) * per).
+    const int per = gridDim.x >> 3;
+    const int lb = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    const int idx = lb * BLOCK + threadIdx.x;
+    unsigned rays = 0, nvisit = 0, ntest = 0;
+    int err = 0;
+    if (lb < nblocks && idx < P.ncell) {
+        int* stk = stk_all + threadIdx.x;
+        const int cell = order ? order[idx] : idx;
+        const CellFrame f = cells[cell];
+        const D3 n = {f.n[0], f.n[1], f.n[2]};
+        const D3 north = {f.north[0], f.north[1], f.north[2]};
+        const float o0 = f.org[0], o1 = f.org[1], o2 = f.org[2];
+        const double a = P.hori_acc;
+        const int L = P.num_hori * P.refine_factor;
+        const bool have = num_tri > 0;
+
+        auto cast = [&](const D3& d) -> bool {
+            rays++;
+            if (!have) return false;
+            Ray r = make_ray(o0, o1, o2, (float)d.x, (float)d.y, (float)d.z, P.tfar);
+            return occluded<COUNT, BLOCK>(nodes, tris, stk, r, nvisit, ntest, err);
+        };
+
+        D3 dir = rodrigues(north, n, P.shift_sin, P.shift_cos);
+        double e = 0.0;
+        double sector = 0.0, svf = 0.0;
+        int ksec = 0, jsec = 0;
+        for (int j = 0; j < L; j++) {
+            const D3 ax = unit_cross(dir, n);
+            double h;
+            if (MODE == 0) {
+                // reference walk (:521-546)
+                bool hit = cast(dir);
+                if (hit) {
+                    while (hit) {
+                        if (e > kPi) {  // a ray past the zenith still hits: not a heightfield
+                            err |= 2;
+                            break;
+                        }
+                        e += 2.0 * a;
+                        dir = rodrigues(dir, ax, P.elev_sin_2ha, P.elev_cos_2ha);
+                        hit = cast(dir);
+                    }
+                    h = e - a;
+                } else {
+                    while (!hit && e > P.elev_thresh) {
+                        e -= 2.0 * a;
+                        dir = rodrigues(dir, ax, -P.elev_sin_2ha, P.elev_cos_2ha);
+                        hit = cast(dir);
+                    }
+                    h = e + a;
+                }
+            } else {
+                // clip-aware climb from the horizontal direction
+                bool hit = cast(dir);
+                if (!hit) {
+                    h = -a;
+                } else {
+                    double ek = 0.0;
+                    D3 dk = dir;
+                    while (hit) {
+                        if (ek > kPi) {
+                            err |= 2;
+                            break;
+                        }
+                        ek += 2.0 * a;
+                        dk = rodrigues(dk, ax, P.elev_sin_2ha, P.elev_cos_2ha);
+                        hit = cast(dk);
+                    }
+                    h = ek - a;
+                }
+            }
+            if (h < 0.0) h = 0.0;  // (:737-741)
+            sector += h;
+            if (++ksec == P.refine_factor) {  // sector mean in degrees (:744-751)
+                horizon[(size_t)jsec * P.ncell + cell] = ((sector / kPi) * 180.0) / (double)P.refine_factor;
+                sector = 0.0;
+                ksec = 0;
+                jsec++;
+            }
+            double s = sin(h);  // SVF (:333-358)
+            if (P.svf_type == 0)
+                svf += (1.0 - s);
+            else if (P.svf_type == 1)
+                svf += (1.0 - (s * s));
+            else
+                svf += (1.0 - (s * s * s));
+            dir = rodrigues(dir, n, -P.azim_sin, P.azim_cos);  // next azimuth (:550-551)
+        }
+        skyview[cell] = svf / (double)L;
+    }
+    unsigned long long wr = wave_sum(rays);
+    unsigned long long wn = COUNT ? wave_sum(nvisit) : 0ull;
+    unsigned long long wt = COUNT ? wave_sum(ntest) : 0ull;
+    int werr = __any(err != 0);
+    if ((threadIdx.x & 63) == 0) {
+        if (wr) atomicAdd(&cnt->rays, wr);
+        if (COUNT) {
+            atomicAdd(&cnt->nodes, wn);
+            atomicAdd(&cnt->tris, wt);
+        }
+        if (werr) atomicOr(&cnt->error, 1ull);
+    }
+}
+
+template <int MODE, bool COUNT, int BLOCK>
+void launch_one(const DeviceBvh& bvh, const CellFrame* cells, const int* order,
+                const WalkParams& p, double* horizon, double* skyview, TraceCounters* cnt,
+                hipStream_t st) {
+    int nb = (p.ncell + BLOCK - 1) / BLOCK;
+    int g = (nb + 7) / 8 * 8;
+    hipLaunchKernelGGL((k_trace<MODE, COUNT, BLOCK>), dim3(g), dim3(BLOCK), 0, st, bvh.nodes,
+                       bvh.tris, bvh.num_tri, cells, order, p, nb, horizon, skyview, cnt);
+}
+
+template <int MODE, bool COUNT>
+void launch_block(int block, const DeviceBvh& bvh, const CellFrame* cells, const int* order,
+                  const WalkParams& p, double* horizon, double* skyview, TraceCounters* cnt,
+                  hipStream_t st) {
+    if (block == 64)
+        launch_one<MODE, COUNT, 64>(bvh, cells, order, p, horizon, skyview, cnt, st);
+    else if (block == 256)
+        launch_one<MODE, COUNT, 256>(bvh, cells, order, p, horizon, skyview, cnt, st);
+    else
+        launch_one<MODE, COUNT, 128>(bvh, cells, order, p, horizon, skyview, cnt, st);
+}
+
+}  // namespace
+
+int launch_trace(const DeviceBvh& bvh, const CellFrame* cells, const int* order,
+                 const WalkParams& p, int mode, int block, bool counters, double* horizon,
+                 double* skyview, TraceCounters* cnt, hipStream_t st, std::string& err) {
+    if (mode == 0) {
+        if (counters)
+            launch_block<0, true>(block, bvh, cells, order, p, horizon, skyview, cnt, st);
+        else
+            launch_block<0, false>(block, bvh, cells, order, p, horizon, skyview, cnt, st);
+    } else {
+        if (counters)
+            launch_block<1, true>(block, bvh, cells, order, p, horizon, skyview, cnt, st);
+        else
+            launch_block<1, false>(block, bvh, cells, order, p, horizon, skyview, cnt, st);
+    }
+    HZ_HIP(hipGetLastError(), err);
+    return 0;
+}
+
+}  // namespace hz

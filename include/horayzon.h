@@ -1,0 +1,143 @@
+/*
+ * horayzon.h -- C-ABI of libhorayzon.so, the MI355X (gfx950) horizon and
+ * sky-view-factor library.
+ *
+ * Plain C types only (pointers, ints, doubles): a Python ctypes stub, a
+ * Cython extern block or a Fortran ISO_C_BINDING interface binds it
+ * directly.  All entry points are synchronous: they return once the results
+ * are in the caller's arrays (or in device memory, for hz_domain_run).
+ *
+ * Error convention: 0 on success, a negative HZ_E* code on failure;
+ * hz_last_error() returns the message of the calling thread's last failure.
+ * On failure the caller's output arrays are filled with NaN, never left
+ * half-written.
+ */
+#ifndef HORAYZON_H
+#define HORAYZON_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HZ_OK 0
+#define HZ_EINVAL (-1)  /* argument out of the reference's documented range */
+#define HZ_EDEVICE (-2) /* HIP runtime error (no device, launch failure, ...) */
+#define HZ_ENOMEM (-3)  /* device or host allocation failed */
+#define HZ_EKERNEL (-4) /* a kernel reported an internal limit (e.g. stack depth) */
+
+/* Search strategies of the elevation-angle search. */
+#define HZ_MODE_WALK 0    /* the reference walk: carried elevation, 0.5 deg steps, -85 deg floor */
+#define HZ_MODE_CLIMB 1   /* clip-aware per-azimuth climb from 0 deg (independent azimuths) */
+
+typedef struct hz_options {
+    int search_mode;      /* HZ_MODE_*; default HZ_MODE_WALK */
+    int leaf_size;        /* BVH leaf size 1..8; 0 = library default */
+    int block_size;       /* trace kernel workgroup size (64, 128, 256); 0 = default */
+    int collect_counters; /* 1: count BVH node visits and triangle tests (slower) */
+    int cell_order;       /* 1: trace cells in Morton order of their position (default), 0: input order */
+    int verbose;          /* 1: print the reference's progress lines to stderr */
+    int reserved[10];
+} hz_options;
+
+typedef struct hz_stats {
+    long long num_rays;          /* occlusion queries cast */
+    long long node_visits;       /* BVH interior nodes fetched (collect_counters only) */
+    long long tri_tests;         /* ray/triangle tests (collect_counters only) */
+    long long num_triangles;     /* triangles in the mesh */
+    long long num_mesh_vertices; /* mesh vertices (cells + added ICON vertices) */
+    long long bvh_nodes;         /* interior BVH nodes */
+    int bvh_depth;               /* deepest interior level (0 = root) */
+    int kernel_error;            /* kernel-side error flags (0 = none) */
+    double t_prologue_ms;        /* host FP64 mesh + ENU + frames */
+    double t_upload_ms;          /* host -> device copies */
+    double t_bvh_ms;             /* device LBVH build (HIP events) */
+    double t_order_ms;           /* device cell ordering (HIP events) */
+    double t_trace_ms;           /* device search + fused sector mean / SVF (HIP events) */
+    double t_download_ms;        /* device -> host copies */
+    double t_total_ms;           /* wall time of the call */
+    double lon0, lat0;           /* ENU origin [rad] */
+    double reserved[8];
+} hz_stats;
+
+/*
+ * Drop-in for the reference entry point
+ *   void horizon_svf_comp(...)   mo_lradtopo_horayzon.h:4-11,
+ *                                mo_lradtopo_horayzon.cpp:562-570
+ * Same argument order, types, units and output layout:
+ *   horizon[j * num_cell + i]  (deg), skyview[i]  (-).
+ * Errors are printed to stderr and the outputs are filled with NaN.
+ */
+void horizon_svf_comp(double* clon, double* clat, double* hsurf, double* vlon, double* vlat,
+                      int* cells_of_vertex, double* horizon, double* skyview, int num_cell,
+                      int num_vertex, int num_hori, int grid_type, double dist_search,
+                      double ray_org_elev, int refine_factor, int svf_type);
+
+/*
+ * horizon_svf_comp with options, a status code and statistics; the
+ * counterpart of the same reference call (mo_lradtopo_horayzon.cpp:562-785),
+ * whose only reporting is the printed timers and ray counts (:604-608,
+ * :424-428, :763-773).  opt and stats may be NULL.
+ */
+int horizon_svf_comp_ex(const double* clon, const double* clat, const double* hsurf,
+                        const double* vlon, const double* vlat, const int* cells_of_vertex,
+                        double* horizon, double* skyview, int num_cell, int num_vertex,
+                        int num_hori, int grid_type, double dist_search, double ray_org_elev,
+                        int refine_factor, int svf_type, const hz_options* opt,
+                        hz_stats* stats);
+
+/* Message of the calling thread's last failure ("" if none). */
+const char* hz_last_error(void);
+
+/* Fill *opt with the library defaults. */
+void hz_default_options(hz_options* opt);
+
+/*
+ * Resident domain: the FP64 prologue (mesh, ENU, per-cell frames) runs once
+ * in hz_domain_create and its FP32 scene and frames stay in device memory;
+ * hz_domain_run then performs one full device pass (LBVH build, cell
+ * ordering, search, sector mean and SVF) with its outputs kept in device
+ * memory until hz_domain_fetch copies them out.  The three calls together
+ * compute exactly what horizon_svf_comp computes (reference
+ * mo_lradtopo_horayzon.cpp:596-761); the split exists so that callers can
+ * time the device pass alone or re-run it with other search settings.
+ */
+typedef struct hz_domain hz_domain;
+
+int hz_domain_create(const double* clon, const double* clat, const double* hsurf,
+                     const double* vlon, const double* vlat, const int* cells_of_vertex,
+                     int num_cell, int num_vertex, int grid_type, double ray_org_elev,
+                     const hz_options* opt, hz_domain** out, hz_stats* stats);
+
+int hz_domain_run(hz_domain* dom, int num_hori, double dist_search, int refine_factor,
+                  int svf_type, const hz_options* opt, hz_stats* stats);
+
+int hz_domain_fetch(hz_domain* dom, double* horizon, double* skyview, hz_stats* stats);
+
+void hz_domain_destroy(hz_domain* dom);
+
+/*
+ * Diagnostics, host only (no device needed): the FP32 ENU scene and per-cell
+ * ray frames that the device pass traces, i.e. the output of the reference's
+ * mesh builder and FP64 prologue (mo_lradtopo_horayzon.cpp:596-722; the
+ * reference's own check of this stage is the printed mesh checksum of
+ * verification.py:171-172).  Pass NULL arrays to obtain the counts only.
+ *   vert32[3 * num_mesh_vertices], tri[3 * num_triangles],
+ *   frame_n / frame_north [3 * num_cell] (FP64), org [3 * num_cell] (FP32),
+ *   lon0_lat0[2] (ENU origin, rad)
+ */
+int hz_scene_export(const double* clon, const double* clat, const double* hsurf,
+                    const double* vlon, const double* vlat, const int* cells_of_vertex,
+                    int num_cell, int num_vertex, int grid_type, double ray_org_elev,
+                    long long* num_triangles, long long* num_mesh_vertices, float* vert32,
+                    int* tri, double* frame_n, double* frame_north, float* org,
+                    double* lon0_lat0);
+
+/* Number of cells / mesh triangles of a resident domain. */
+int hz_domain_num_cell(const hz_domain* dom);
+long long hz_domain_num_triangles(const hz_domain* dom);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HORAYZON_H */

@@ -1,0 +1,101 @@
+"""GPU parity: libhorayzon.so (through its C-ABI) against the CPU oracle.
+
+The oracle restates the reference walk (mo_lradtopo_horayzon.cpp:490-555)
+with the same FP64 direction arithmetic and the same FP32 watertight test,
+so the bar is bit-exact horizon values; the SVF differs only through the
+device's sin() (tolerance 1e-12).
+"""
+
+import numpy as np
+import pytest
+
+import oracle
+from horayzon_extpar_amd import synthetic
+from horayzon_extpar_amd.horizon_svf import ResidentDomain, horizon_svf_comp_py
+
+pytestmark = pytest.mark.gpu
+
+SVF_TOL = 1e-12
+
+
+def _report(name, h_gpu, h_orc):
+    diff = h_gpu != h_orc
+    print(f"[{name}] angles={h_gpu.size} differing={int(diff.sum())} "
+          f"max|dh|={float(np.abs(h_gpu - h_orc).max()):.3g} deg", flush=True)
+
+
+@pytest.mark.parametrize("grid_type", [1, 0])
+@pytest.mark.parametrize("svf_type", [2, 0])
+def test_c1_walk_bit_exact(c1, grid_type, svf_type):
+    print(f"case c1 walk gt={grid_type} svf={svf_type}", flush=True)
+    st = {}
+    h, s = horizon_svf_comp_py(*c1.args(), 24, grid_type, 40000.0, 0.2, 1, svf_type,
+                               stats=st)
+    ho, so, sto = oracle.horizon_svf(c1, grid_type=grid_type, svf_type=svf_type)
+    _report(f"c1 gt{grid_type}", h, ho)
+    assert np.array_equal(h, ho)
+    assert np.max(np.abs(s - so)) <= SVF_TOL
+    assert st["num_rays"] == sto.num_rays
+
+
+def test_c1_refine_factor_10(c1):
+    print("case c1 rf=10 svf=1", flush=True)
+    h, s = horizon_svf_comp_py(*c1.args(), 24, 1, 40000.0, 0.2, 10, 1)
+    ho, so, _ = oracle.horizon_svf(c1, grid_type=1, refine_factor=10, svf_type=1)
+    _report("c1 rf10", h, ho)
+    assert np.array_equal(h, ho)
+    assert np.max(np.abs(s - so)) <= SVF_TOL
+
+
+def test_c1_climb_mode(c1):
+    print("case c1 climb", flush=True)
+    h, s = horizon_svf_comp_py(*c1.args(), 24, 1, 40000.0, 0.2, 1, 2, search_mode=1)
+    ho, so, _ = oracle.horizon_svf(c1, grid_type=1, mode=1)
+    _report("c1 climb", h, ho)
+    assert np.array_equal(h, ho)
+    assert np.max(np.abs(s - so)) <= SVF_TOL
+
+
+@pytest.mark.parametrize("leaf", [1, 8])
+def test_leaf_size_independent(c1, leaf):
+    # the occlusion answer is the OR over all triangles: any tree gives the same bits
+    print(f"case c1 leaf={leaf}", flush=True)
+    h4, s4 = horizon_svf_comp_py(*c1.args(), 24, 1, 40000.0, 0.2, 1, 2)
+    h, s = horizon_svf_comp_py(*c1.args(), 24, 1, 40000.0, 0.2, 1, 2, leaf_size=leaf,
+                               block_size=64)
+    assert np.array_equal(h, h4) and np.array_equal(s, s4)
+
+
+def test_flat_plain():
+    print("case flat plain", flush=True)
+    d = synthetic.lattice_domain(40.0, 40.0, 2.0)
+    h, s = horizon_svf_comp_py(*d.args(), 24, 1, 40000.0, 0.2, 1, 2)
+    assert np.all(h == 0.0) and np.all(s == 1.0)
+
+
+def test_c2_sampled_and_invariants(c2):
+    print("case c2 sampled", flush=True)
+    with ResidentDomain(*c2.args(), grid_type=1, ray_org_elev=0.2) as dom:
+        st = dom.run(24, 40000.0, 1, 2)
+        h, s = dom.fetch()
+        st2 = dom.run(24, 40000.0, 1, 2)
+        h2, s2 = dom.fetch()
+    print(f"c2 rays={st['num_rays']} trace_ms={st['t_trace_ms']:.2f} "
+          f"bvh_ms={st['t_bvh_ms']:.2f}", flush=True)
+    assert np.array_equal(h, h2) and np.array_equal(s, s2)  # deterministic
+    assert st["num_rays"] == st2["num_rays"]
+    # lattice invariant at rf = 1: every value is 0 or 0.25 + 0.5 k
+    k = (h - 0.25) / 0.5
+    on_lattice = (h == 0.0) | (np.abs(k - np.round(k)) < 1e-9)
+    assert on_lattice.all()
+    assert np.all((s > 0.0) & (s <= 1.0))
+    # oracle on three contiguous cell ranges
+    ho = np.full_like(h, np.nan)
+    so = np.full_like(s, np.nan)
+    n = c2.ncell
+    for b in (0, n // 2, n - 700):
+        oracle.horizon_svf(c2, grid_type=1, cell_begin=b, cell_end=b + 700, out=(ho, so))
+    m = ~np.isnan(so)
+    _report("c2 sampled", h[:, m], ho[:, m])
+    assert np.array_equal(h[:, m], ho[:, m])
+    assert np.max(np.abs(s[m] - so[m])) <= SVF_TOL
