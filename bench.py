@@ -37,6 +37,8 @@ def main():
     ap.add_argument("--mode", default="walk", choices=["walk", "climb"])
     ap.add_argument("--leaf", type=int, default=0)
     ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--kernel", type=int, default=0, help="0 persistent, 1 one lane per cell")
+    ap.add_argument("--refill", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -56,7 +58,8 @@ def main():
     nhori, dist, rf, svf = 24, 40000.0, 1, 2
     print(f"[bench] workload {args.config}: ncell={dom_in.ncell} nvert={dom_in.nvert}",
           file=sys.stderr, flush=True)
-    opts = dict(search_mode=mode, leaf_size=args.leaf or None, block_size=args.block or None)
+    opts = dict(search_mode=mode, leaf_size=args.leaf or None, block_size=args.block or None,
+                trace_kernel=args.kernel, refill=args.refill or None)
     with ResidentDomain(*dom_in.args(), grid_type=1, ray_org_elev=0.2) as dom:
         cst = dom.create_stats
         for _ in range(args.warmup):
@@ -100,7 +103,9 @@ def main():
         "config": {"workload": f"{args.config}: 600x400 km @1 km ICON-like grid, "
                                f"{ncell} cells, {cnt['num_triangles']} triangles (grid_type 1)",
                    "num_hori": nhori, "dist_search_m": dist, "refine_factor": rf,
-                   "svf_type": svf, "search": args.mode},
+                   "svf_type": svf, "search": args.mode,
+                   "trace_kernel": ["persistent", "lane-per-cell", "persistent-4w",
+                                    "persistent-5w"][args.kernel]},
         "rays_per_s": rays / (tr * 1e-3),
         "rays_per_cell_sector": rays / (ncell * nhori),
         "ms_trace": tr, "ms_bvh": float(np.mean(t_bvh)), "ms_order": float(np.mean(t_ord)),
@@ -120,6 +125,7 @@ def cpu_baseline(dom_in, budget_s, threads):
     """The oracle (C++ port of the reference walk, FP32 CPU BVH, std::thread
     pool with dynamic chunks) on a time-budgeted contiguous block of cells at
     the domain centre."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as orc
 
     nt = threads or min(os.cpu_count() or 1, 16)

@@ -30,7 +30,8 @@ class HzOptions(ctypes.Structure):
     _fields_ = [("search_mode", ctypes.c_int), ("leaf_size", ctypes.c_int),
                 ("block_size", ctypes.c_int), ("collect_counters", ctypes.c_int),
                 ("cell_order", ctypes.c_int), ("verbose", ctypes.c_int),
-                ("reserved", ctypes.c_int * 10)]
+                ("trace_kernel", ctypes.c_int), ("refill", ctypes.c_int),
+                ("reserved", ctypes.c_int * 8)]
 
 
 class HzStats(ctypes.Structure):

@@ -98,7 +98,8 @@ struct hz_domain {
     double* d_skyview = nullptr;
     int out_nhori = 0;  // horizon rows held by d_horizon
     int last_nhori = 0; // rows written by the last run
-    TraceCounters* d_cnt = nullptr;
+    TraceState* d_state = nullptr;
+    double* d_spill = nullptr;
     LbvhWork work;
     DeviceBvh bvh;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -117,6 +118,8 @@ void hz_default_options(hz_options* o) {
     o->collect_counters = 0;
     o->cell_order = 1;
     o->verbose = 0;
+    o->trace_kernel = 0;
+    o->refill = 40;
 }
 
 void hz_domain_destroy(hz_domain* d) {
@@ -124,7 +127,8 @@ void hz_domain_destroy(hz_domain* d) {
     (void)hipSetDevice(d->device);
     if (d->stream) (void)hipStreamSynchronize(d->stream);
     void* bufs[] = {d->d_vert,   d->d_tri,     d->d_cells,   d->d_order, d->d_nodes,
-                    d->d_tris,   d->d_horizon, d->d_skyview, d->d_cnt};
+                    d->d_tris,   d->d_horizon, d->d_skyview, d->d_state,
+                    d->d_spill};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     lbvh_free(d->work);
@@ -226,7 +230,8 @@ int hz_domain_create(const double* clon, const double* clat, const double* hsurf
     DCHK(hipMalloc(&d->d_nodes, std::max<long long>(1, d->ntri - 1) * sizeof(BvhNode)));
     DCHK(hipMalloc(&d->d_tris, std::max<long long>(1, d->ntri) * 3 * sizeof(float4)));
     DCHK(hipMalloc(&d->d_skyview, (size_t)num_cell * sizeof(double)));
-    DCHK(hipMalloc(&d->d_cnt, sizeof(TraceCounters)));
+    DCHK(hipMalloc(&d->d_state, sizeof(TraceState)));
+    DCHK(hipMalloc(&d->d_spill, trace_spill_doubles(num_cell) * sizeof(double)));
     if (nv) DCHK(hipMemcpyAsync(d->d_vert, hs.vert32.data(), nv * sizeof(float), hipMemcpyHostToDevice, d->stream));
     if (nt) DCHK(hipMemcpyAsync(d->d_tri, hs.tri.data(), nt * sizeof(int), hipMemcpyHostToDevice, d->stream));
     DCHK(hipMemcpyAsync(d->d_cells, hs.frame.data(), (size_t)num_cell * sizeof(CellFrame),
@@ -288,7 +293,6 @@ int hz_domain_run(hz_domain* d, int num_hori, double dist_search, int refine_fac
     }
     std::string err;
     hipStream_t st = d->stream;
-    RCHK(hipMemsetAsync(d->d_cnt, 0, sizeof(TraceCounters), st));
     RCHK(hipEventRecord(d->ev[0], st));
     d->bvh.nodes = d->d_nodes;
     d->bvh.tris = d->d_tris;
@@ -304,12 +308,17 @@ int hz_domain_run(hz_domain* d, int num_hori, double dist_search, int refine_fac
     }
     RCHK(hipEventRecord(d->ev[2], st));
     WalkParams p = make_walk_params(num_hori, refine_factor, svf_type, dist_search, d->ncell);
-    rc = launch_trace(d->bvh, d->d_cells, order, p, o.search_mode, block, o.collect_counters != 0,
-                      d->d_horizon, d->d_skyview, d->d_cnt, st, err);
+    if (o.trace_kernel < 0 || o.trace_kernel > 3)
+        return fail(HZ_EINVAL, "trace_kernel must be in [0, 3]");
+    int refill = o.refill ? o.refill : 40;
+    if (refill < 1 || refill > 64) return fail(HZ_EINVAL, "refill must be in [1, 64]");
+    rc = launch_trace(d->bvh, d->d_cells, order, p, o.search_mode, o.trace_kernel, block, refill,
+                      o.collect_counters != 0, d->d_horizon, d->d_skyview, d->d_state, d->d_spill, st,
+                      err);
     if (rc) return fail(HZ_EDEVICE, err);
     RCHK(hipEventRecord(d->ev[3], st));
     TraceCounters c;
-    RCHK(hipMemcpyAsync(&c, d->d_cnt, sizeof(c), hipMemcpyDeviceToHost, st));
+    RCHK(hipMemcpyAsync(&c, &d->d_state->cnt, sizeof(c), hipMemcpyDeviceToHost, st));
     RCHK(hipStreamSynchronize(st));
     float ms_bvh = 0, ms_ord = 0, ms_trace = 0;
     RCHK(hipEventElapsedTime(&ms_bvh, d->ev[0], d->ev[1]));

@@ -25,24 +25,31 @@ struct alignas(16) CellFrame {
 };
 static_assert(sizeof(CellFrame) == 64, "CellFrame layout");
 
-// BVH2 interior node: the conservative boxes of both children and the two
-// child references, 64 bytes = four float4 loads.
-//   ref >= 0 : interior node index
-//   ref <  0 : leaf; ~ref = first | ((count - 1) << 28), triangles in leaf order
+// BVH2 interior node, 128 bytes (one L2 line): the conservative boxes of
+// both children, the two child references and, for a child that is a
+// single-triangle leaf, that triangle's FP32 vertices inline, so one fetch
+// brings everything a step needs.
+//   ref >= 0 : interior node index in bits 0..27; bit 30 set when that node
+//              holds an inline triangle (its second 64 bytes must be read)
+//   ref <  0 : leaf; ~ref = first | ((count - 1) << 28), triangles in leaf
+//              order; a leaf of count 1 is the inline triangle of its slot
 struct alignas(16) BvhNode {
     float lx0, lx1, hx0, hx1;
     float ly0, ly1, hy0, hy1;
     float lz0, lz1, hz0, hz1;
-    int r0, r1, pad0, pad1;
+    int r0, r1;
+    float tri[18];  // slot 0: tri[0..8], slot 1: tri[9..17] (v0, v1, v2 xyz)
 };
-static_assert(sizeof(BvhNode) == 64, "BvhNode layout");
+static_assert(sizeof(BvhNode) == 128, "BvhNode layout");
 
 constexpr int kLeafFirstBits = 28;
 constexpr int kLeafFirstMask = (1 << kLeafFirstBits) - 1;
+constexpr int kWideBit = 1 << 30;
 constexpr int kMaxLeafSize = 8;
 
-// Triangles in leaf order: 3 float4 per triangle
-//   {v0.x, v0.y, v0.z, v1.x}, {v1.y, v1.z, v2.x, v2.y}, {v2.z, id, 0, 0}
+// Triangles in leaf order (used by leaves of more than one triangle):
+// 3 float4 per triangle {v0.x, v0.y, v0.z, v1.x}, {v1.y, v1.z, v2.x, v2.y},
+// {v2.z, id, 0, 0}
 struct DeviceBvh {
     BvhNode* nodes = nullptr;
     float4* tris = nullptr;
@@ -127,9 +134,26 @@ struct TraceCounters {
     unsigned long long error;
 };
 
+// Device-side state of one trace launch: counters plus the work queue of
+// the persistent kernel (one cell counter per XCD, 128 bytes apart).
+constexpr int kQueueStride = 32;
+struct TraceState {
+    TraceCounters cnt;
+    int pad[24];
+    int queue[8 * kQueueStride];
+};
+
+// kernel: 0 = persistent lanes with per-XCD work queues (default),
+//         1 = one lane per cell, azimuths in lock step
+// refill: the persistent kernel leaves its traversal loop to refill lanes
+//         once fewer than `refill` of the 64 lanes hold a live ray
+// spill: per-lane FP64 search state of the persistent kernel, at least
+// trace_spill_doubles(...) doubles
 int launch_trace(const DeviceBvh& bvh, const CellFrame* cells, const int* order,
-                 const WalkParams& p, int mode, int block_size, bool counters, double* horizon,
-                 double* skyview, TraceCounters* dcount, hipStream_t stream, std::string& err);
+                 const WalkParams& p, int mode, int kernel, int block_size, int refill,
+                 bool counters, double* horizon, double* skyview, TraceState* state,
+                 double* spill, hipStream_t stream, std::string& err);
+size_t trace_spill_doubles(int ncell);
 
 }  // namespace hz
 

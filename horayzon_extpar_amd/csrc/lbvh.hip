@@ -169,28 +169,38 @@ __global__ __launch_bounds__(kBlock) void k_zero(int* __restrict__ p, int n) {
     if (i < n) p[i] = 0;
 }
 
-// Leaf boxes were written in original triangle order; the refit reads them
-// through the sorted permutation.
+// Bottom-up boxes.  One thread per leaf climbs while it is the second
+// arrival at a node (arrival counter per node).  Leaf boxes come from the
+// previous kernel.  The box a thread writes is read by the thread that
+// completes the parent: when the parent's leaf range lies inside this
+// workgroup's leaf chunk both threads belong to this workgroup and a
+// workgroup-scope fence orders the write and the read; only nodes whose
+// range crosses a chunk border (about two per chunk and level) pay for an
+// agent-scope fence, which on gfx950 writes back / invalidates the XCD's L2.
 __global__ __launch_bounds__(kBlock) void k_refit(const int* __restrict__ sorted, int n,
                                                  float4* __restrict__ lo, float4* __restrict__ hi,
                                                  const int* __restrict__ child,
+                                                 const int* __restrict__ range,
                                                  const int* __restrict__ parent,
                                                  int* __restrict__ flags) {
     int leaf = blockIdx.x * kBlock + threadIdx.x;
     if (leaf >= n) return;
+    const int chunk_lo = blockIdx.x * kBlock, chunk_hi = chunk_lo + kBlock - 1;
     int p = parent[leaf];
+    bool wrote = false;
     // a valid tree has < 128 levels (63 code bits + 32 index bits); the cap
     // only bounds the loop
     for (int guard = 0; p >= 0 && guard < 256; guard++) {
-        __threadfence();
+        const bool local = range[2 * p] >= chunk_lo && range[2 * p + 1] <= chunk_hi;
+        if (wrote) {  // release the box written in the previous iteration
+            if (local) __threadfence_block(); else __threadfence();
+        }
         if (atomicAdd(&flags[p], 1) == 0) return;  // first arrival: sibling not ready
-        __threadfence();
+        if (local) __threadfence_block(); else __threadfence();  // acquire
         float4 l[2], h[2];
         for (int s = 0; s < 2; s++) {
             int c = child[2 * p + s];
             int slot = c < 0 ? sorted[~c] : n + c;
-            // the agent-scope fence above orders this read after the
-            // sibling's write (which preceded its own fence and atomic)
             l[s] = lo[slot];
             h[s] = hi[slot];
         }
@@ -198,6 +208,7 @@ __global__ __launch_bounds__(kBlock) void k_refit(const int* __restrict__ sorted
                                 fminf(l[0].z, l[1].z), 0.0f);
         hi[n + p] = make_float4(fmaxf(h[0].x, h[1].x), fmaxf(h[0].y, h[1].y),
                                 fmaxf(h[0].z, h[1].z), 0.0f);
+        wrote = true;
         p = parent[n + p];
     }
 }
@@ -205,16 +216,30 @@ __global__ __launch_bounds__(kBlock) void k_refit(const int* __restrict__ sorted
 __device__ __forceinline__ float grow_lo(float v) { return v - (1.0f + 1e-5f * fabsf(v)); }
 __device__ __forceinline__ float grow_hi(float v) { return v + (1.0f + 1e-5f * fabsf(v)); }
 
+__device__ __forceinline__ void put_tri(float* dst, const float* __restrict__ vert,
+                                        const int* __restrict__ tri, int t) {
+    for (int k = 0; k < 3; k++) {
+        int v = tri[3 * t + k];
+        dst[3 * k] = vert[3 * v];
+        dst[3 * k + 1] = vert[3 * v + 1];
+        dst[3 * k + 2] = vert[3 * v + 2];
+    }
+}
+
 __global__ __launch_bounds__(kBlock) void k_pack(const int* __restrict__ sorted, int n,
                                                 const float4* __restrict__ lo,
                                                 const float4* __restrict__ hi,
                                                 const int* __restrict__ child,
                                                 const int* __restrict__ range, int leaf_size,
+                                                const float* __restrict__ vert,
+                                                const int* __restrict__ tri,
                                                 BvhNode* __restrict__ nodes) {
     int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n - 1) return;
     float4 l[2], h[2];
     int ref[2];
+    BvhNode nd;
+    for (int k = 0; k < 18; k++) nd.tri[k] = 0.0f;
     for (int s = 0; s < 2; s++) {
         int c = child[2 * i + s];
         if (c < 0) {
@@ -222,16 +247,21 @@ __global__ __launch_bounds__(kBlock) void k_pack(const int* __restrict__ sorted,
             int slot = sorted[leaf];
             l[s] = lo[slot];
             h[s] = hi[slot];
-            ref[s] = ~leaf;  // count 1
+            ref[s] = ~leaf;  // count 1: inline triangle
+            put_tri(&nd.tri[9 * s], vert, tri, slot);
         } else {
             l[s] = lo[n + c];
             h[s] = hi[n + c];
             int first = range[2 * c], last = range[2 * c + 1];
             int cnt = last - first + 1;
-            ref[s] = (cnt <= leaf_size) ? ~(first | ((cnt - 1) << kLeafFirstBits)) : c;
+            if (cnt <= leaf_size) {
+                ref[s] = ~(first | ((cnt - 1) << kLeafFirstBits));
+            } else {
+                bool wide = child[2 * c] < 0 || child[2 * c + 1] < 0;
+                ref[s] = c | (wide ? kWideBit : 0);
+            }
         }
     }
-    BvhNode nd;
     nd.lx0 = grow_lo(l[0].x);
     nd.lx1 = grow_lo(l[1].x);
     nd.hx0 = grow_hi(h[0].x);
@@ -246,13 +276,12 @@ __global__ __launch_bounds__(kBlock) void k_pack(const int* __restrict__ sorted,
     nd.hz1 = grow_hi(h[1].z);
     nd.r0 = ref[0];
     nd.r1 = ref[1];
-    nd.pad0 = 0;
-    nd.pad1 = 0;
     nodes[i] = nd;
 }
 
-// One triangle: both children of the root are the same leaf.
+// One triangle: both children of the root are that triangle, inline.
 __global__ void k_pack_single(const float4* __restrict__ lo, const float4* __restrict__ hi,
+                              const float* __restrict__ vert, const int* __restrict__ tri,
                               BvhNode* __restrict__ nodes) {
     if (threadIdx.x != 0) return;
     float4 l = lo[0], h = hi[0];
@@ -264,7 +293,8 @@ __global__ void k_pack_single(const float4* __restrict__ lo, const float4* __res
     nd.hy0 = nd.hy1 = grow_hi(h.y);
     nd.hz0 = nd.hz1 = grow_hi(h.z);
     nd.r0 = nd.r1 = ~0;
-    nd.pad0 = nd.pad1 = 0;
+    put_tri(&nd.tri[0], vert, tri, 0);
+    put_tri(&nd.tri[9], vert, tri, 0);
     nodes[0] = nd;
 }
 
@@ -398,8 +428,8 @@ int lbvh_build(LbvhWork& w, const float* vert, const int* tri, int n, int leaf_s
     hipLaunchKernelGGL(k_tri_bounds, dim3(grid(n)), dim3(kBlock), 0, st, vert, tri, n, w.aabb_lo,
                        w.aabb_hi, w.bounds);
     if (n == 1) {
-        hipLaunchKernelGGL(k_pack_single, dim3(1), dim3(64), 0, st, w.aabb_lo, w.aabb_hi,
-                           out.nodes);
+        hipLaunchKernelGGL(k_pack_single, dim3(1), dim3(64), 0, st, w.aabb_lo, w.aabb_hi, vert,
+                           tri, out.nodes);
         hipLaunchKernelGGL(k_morton, dim3(1), dim3(kBlock), 0, st, w.aabb_lo, w.aabb_hi, n,
                            w.bounds, w.keys[0], w.vals[0]);
         hipLaunchKernelGGL(k_tris, dim3(1), dim3(kBlock), 0, st, w.vals[0], n, vert, tri,
@@ -420,9 +450,9 @@ int lbvh_build(LbvhWork& w, const float* vert, const int* tri, int n, int leaf_s
                        w.range, w.parent);
     hipLaunchKernelGGL(k_zero, dim3(grid(n - 1)), dim3(kBlock), 0, st, w.flags, n - 1);
     hipLaunchKernelGGL(k_refit, dim3(grid(n)), dim3(kBlock), 0, st, sorted, n, w.aabb_lo,
-                       w.aabb_hi, w.child, w.parent, w.flags);
+                       w.aabb_hi, w.child, w.range, w.parent, w.flags);
     hipLaunchKernelGGL(k_pack, dim3(grid(n - 1)), dim3(kBlock), 0, st, sorted, n, w.aabb_lo,
-                       w.aabb_hi, w.child, w.range, leaf_size, out.nodes);
+                       w.aabb_hi, w.child, w.range, leaf_size, vert, tri, out.nodes);
     hipLaunchKernelGGL(k_tris, dim3(grid(n)), dim3(kBlock), 0, st, sorted, n, vert, tri,
                        out.tris);
     out.num_nodes = n - 1;

@@ -111,11 +111,11 @@ __device__ __forceinline__ Ray make_ray(float o0, float o1, float o2, float d0, 
 }
 
 // Watertight two-sided test; operation order fixed (no contraction).
-__device__ __forceinline__ bool tri_hit(const Ray& r, float4 t0, float4 t1, float4 t2) {
-    // vertices: v0 = t0.xyz, v1 = (t0.w, t1.x, t1.y), v2 = (t1.z, t1.w, t2.x)
-    float A0 = t0.x - r.o0, A1 = t0.y - r.o1, A2 = t0.z - r.o2;
-    float B0 = t0.w - r.o0, B1 = t1.x - r.o1, B2 = t1.y - r.o2;
-    float C0 = t1.z - r.o0, C1 = t1.w - r.o1, C2 = t2.x - r.o2;
+__device__ __forceinline__ bool tri_hit(const Ray& r, float v0x, float v0y, float v0z, float v1x,
+                                        float v1y, float v1z, float v2x, float v2y, float v2z) {
+    float A0 = v0x - r.o0, A1 = v0y - r.o1, A2 = v0z - r.o2;
+    float B0 = v1x - r.o0, B1 = v1y - r.o1, B2 = v1z - r.o2;
+    float C0 = v2x - r.o0, C1 = v2y - r.o1, C2 = v2z - r.o2;
     float Akx = sel3(A0, A1, A2, r.kx), Aky = sel3(A0, A1, A2, r.ky), Akz = sel3(A0, A1, A2, r.kz);
     float Bkx = sel3(B0, B1, B2, r.kx), Bky = sel3(B0, B1, B2, r.ky), Bkz = sel3(B0, B1, B2, r.kz);
     float Ckx = sel3(C0, C1, C2, r.kx), Cky = sel3(C0, C1, C2, r.ky), Ckz = sel3(C0, C1, C2, r.kz);
@@ -155,18 +155,96 @@ __device__ __forceinline__ bool leaf_hit(const Ray& r, const float4* __restrict_
         const float4* t = tris + 3 * (first + k);
         float4 t0 = t[0], t1 = t[1], t2 = t[2];
         if (COUNT) ntest++;
-        if (tri_hit(r, t0, t1, t2)) return true;
+        if (tri_hit(r, t0.x, t0.y, t0.z, t0.w, t1.x, t1.y, t1.z, t1.w, t2.x)) return true;
     }
     return false;
 }
 
-// Any-hit traversal; nearer child first.  Returns the occlusion answer.
+// One step of the any-hit traversal (nearer child first): fetch node `node`,
+// test both child boxes, test the triangles of hit leaf children at once,
+// then descend or pop.  Returns 0 while the query is open, 1 on a hit, 2
+// when the stack empties without one.  The stack keeps its first kLdsStack
+// entries in LDS at stk[k * BLOCK] and the rest in the private array ovf.
+template <bool COUNT, int BLOCK>
+__device__ __forceinline__ int trav_step(const BvhNode* __restrict__ nodes,
+                                         const float4* __restrict__ tris, int* stk, int* ovf,
+                                         const Ray& r, int& node, int& sp, unsigned& nvisit,
+                                         unsigned& ntest, int& err) {
+    const float4* np = reinterpret_cast<const float4*>(nodes + (node & kLeafFirstMask));
+    const float4 a = np[0], b = np[1], c = np[2], d = np[3];
+    float4 e4 = make_float4(0.f, 0.f, 0.f, 0.f), f4 = e4, g4 = e4, h4 = e4;
+    if (node & kWideBit) {  // inline triangle(s): second half of the node
+        e4 = np[4];
+        f4 = np[5];
+        g4 = np[6];
+        h4 = np[7];
+    }
+    if (COUNT) nvisit++;
+    // child boxes: a = (lx0, lx1, hx0, hx1), b = (ly0, ly1, hy0, hy1), c = (lz0, lz1, hz0, hz1)
+    float x0l = __builtin_fmaf(a.x, r.ix, -r.oix), x0h = __builtin_fmaf(a.z, r.ix, -r.oix);
+    float x1l = __builtin_fmaf(a.y, r.ix, -r.oix), x1h = __builtin_fmaf(a.w, r.ix, -r.oix);
+    float y0l = __builtin_fmaf(b.x, r.iy, -r.oiy), y0h = __builtin_fmaf(b.z, r.iy, -r.oiy);
+    float y1l = __builtin_fmaf(b.y, r.iy, -r.oiy), y1h = __builtin_fmaf(b.w, r.iy, -r.oiy);
+    float z0l = __builtin_fmaf(c.x, r.iz, -r.oiz), z0h = __builtin_fmaf(c.z, r.iz, -r.oiz);
+    float z1l = __builtin_fmaf(c.y, r.iz, -r.oiz), z1h = __builtin_fmaf(c.w, r.iz, -r.oiz);
+    float tmin0 = fmaxf(fmaxf(fminf(x0l, x0h), fminf(y0l, y0h)), fmaxf(fminf(z0l, z0h), -1.0f));
+    float tmax0 = fminf(fminf(fmaxf(x0l, x0h), fmaxf(y0l, y0h)), fminf(fmaxf(z0l, z0h), r.tfar_box));
+    float tmin1 = fmaxf(fmaxf(fminf(x1l, x1h), fminf(y1l, y1h)), fmaxf(fminf(z1l, z1h), -1.0f));
+    float tmax1 = fminf(fminf(fmaxf(x1l, x1h), fmaxf(y1l, y1h)), fminf(fmaxf(z1l, z1h), r.tfar_box));
+    bool h0 = tmin0 <= tmax0;
+    bool h1 = tmin1 <= tmax1;
+    const int r0 = __float_as_int(d.x), r1 = __float_as_int(d.y);
+    if (h0 && r0 < 0) {
+        if (r0 == ~(~r0 & kLeafFirstMask)) {  // single triangle, inline in slot 0
+            if (COUNT) ntest++;
+            if (tri_hit(r, d.z, d.w, e4.x, e4.y, e4.z, e4.w, f4.x, f4.y, f4.z)) return 1;
+        } else if (leaf_hit<COUNT>(r, tris, r0, ntest)) {
+            return 1;
+        }
+        h0 = false;
+    }
+    if (h1 && r1 < 0) {
+        if (r1 == ~(~r1 & kLeafFirstMask)) {  // single triangle, inline in slot 1
+            if (COUNT) ntest++;
+            if (tri_hit(r, f4.w, g4.x, g4.y, g4.z, g4.w, h4.x, h4.y, h4.z, h4.w)) return 1;
+        } else if (leaf_hit<COUNT>(r, tris, r1, ntest)) {
+            return 1;
+        }
+        h1 = false;
+    }
+    if (h0 && h1) {
+        bool first0 = tmin0 <= tmin1;
+        int nearc = first0 ? r0 : r1;
+        int farc = first0 ? r1 : r0;
+        if (sp >= kStackMax) {
+            err |= 1;
+            return 1;
+        }
+        if (sp < kLdsStack)
+            stk[sp * BLOCK] = farc;
+        else
+            ovf[sp - kLdsStack] = farc;
+        sp++;
+        node = nearc;
+    } else if (h0) {
+        node = r0;
+    } else if (h1) {
+        node = r1;
+    } else {
+        if (sp == 0) return 2;
+        sp--;
+        node = sp < kLdsStack ? stk[sp * BLOCK] : ovf[sp - kLdsStack];
+    }
+    return 0;
+}
+
+// Whole any-hit query.
 template <bool COUNT, int BLOCK>
 __device__ __forceinline__ bool occluded(const BvhNode* __restrict__ nodes,
                                          const float4* __restrict__ tris, int* stk, const Ray& r,
                                          unsigned& nvisit, unsigned& ntest, int& err) {
     int ovf[kStackMax - kLdsStack];
-    int node = 0;
+    int node = kWideBit;  // the root is read in full (128 bytes)
     int sp = 0;
     for (int guard = 0;; guard++) {
         // a BVH of n triangles has n - 1 interior nodes; more fetches than
@@ -175,55 +253,8 @@ __device__ __forceinline__ bool occluded(const BvhNode* __restrict__ nodes,
             err |= 4;
             return true;
         }
-        const float4* np = reinterpret_cast<const float4*>(nodes + node);
-        float4 a = np[0], b = np[1], c = np[2];
-        int4 q = reinterpret_cast<const int4*>(np)[3];
-        if (COUNT) nvisit++;
-        // child boxes: a = (lx0, lx1, hx0, hx1), b = (ly0, ly1, hy0, hy1), c = (lz0, lz1, hz0, hz1)
-        float x0l = __builtin_fmaf(a.x, r.ix, -r.oix), x0h = __builtin_fmaf(a.z, r.ix, -r.oix);
-        float x1l = __builtin_fmaf(a.y, r.ix, -r.oix), x1h = __builtin_fmaf(a.w, r.ix, -r.oix);
-        float y0l = __builtin_fmaf(b.x, r.iy, -r.oiy), y0h = __builtin_fmaf(b.z, r.iy, -r.oiy);
-        float y1l = __builtin_fmaf(b.y, r.iy, -r.oiy), y1h = __builtin_fmaf(b.w, r.iy, -r.oiy);
-        float z0l = __builtin_fmaf(c.x, r.iz, -r.oiz), z0h = __builtin_fmaf(c.z, r.iz, -r.oiz);
-        float z1l = __builtin_fmaf(c.y, r.iz, -r.oiz), z1h = __builtin_fmaf(c.w, r.iz, -r.oiz);
-        float tmin0 = fmaxf(fmaxf(fminf(x0l, x0h), fminf(y0l, y0h)), fmaxf(fminf(z0l, z0h), -1.0f));
-        float tmax0 = fminf(fminf(fmaxf(x0l, x0h), fmaxf(y0l, y0h)), fminf(fmaxf(z0l, z0h), r.tfar_box));
-        float tmin1 = fmaxf(fmaxf(fminf(x1l, x1h), fminf(y1l, y1h)), fmaxf(fminf(z1l, z1h), -1.0f));
-        float tmax1 = fminf(fminf(fmaxf(x1l, x1h), fmaxf(y1l, y1h)), fminf(fmaxf(z1l, z1h), r.tfar_box));
-        bool h0 = tmin0 <= tmax0;
-        bool h1 = tmin1 <= tmax1;
-        int r0 = q.x, r1 = q.y;
-        if (h0 && r0 < 0) {
-            if (leaf_hit<COUNT>(r, tris, r0, ntest)) return true;
-            h0 = false;
-        }
-        if (h1 && r1 < 0) {
-            if (leaf_hit<COUNT>(r, tris, r1, ntest)) return true;
-            h1 = false;
-        }
-        if (h0 && h1) {
-            bool first0 = tmin0 <= tmin1;
-            int nearc = first0 ? r0 : r1;
-            int farc = first0 ? r1 : r0;
-            if (sp >= kStackMax) {
-                err = 1;
-                return true;
-            }
-            if (sp < kLdsStack)
-                stk[sp * BLOCK] = farc;
-            else
-                ovf[sp - kLdsStack] = farc;
-            sp++;
-            node = nearc;
-        } else if (h0) {
-            node = r0;
-        } else if (h1) {
-            node = r1;
-        } else {
-            if (sp == 0) return false;
-            sp--;
-            node = sp < kLdsStack ? stk[sp * BLOCK] : ovf[sp - kLdsStack];
-        }
+        int s = trav_step<COUNT, BLOCK>(nodes, tris, stk, ovf, r, node, sp, nvisit, ntest, err);
+        if (s) return s == 1;
     }
 }
 
@@ -348,10 +379,356 @@ __global__ __launch_bounds__(BLOCK) void k_trace(const BvhNode* __restrict__ nod
     }
 }
 
+// ---------------------------------------------------------------------------
+// Persistent kernel
+//
+// Every lane runs the elevation search of its current cell as a small state
+// machine and keeps one occlusion query in flight.  The wave alternates
+// between a traversal phase (one BVH step per iteration for every lane with
+// a live query) and a refill phase entered once fewer than `refill` lanes
+// still have a live query: lanes whose query ended consume its answer,
+// advance their search (next elevation step, next azimuth, or the end of the
+// cell with its sector means and SVF written out) and issue their next query;
+// lanes whose cell is finished take the next cell from the wave's chunk of
+// the Morton-ordered cell list.  Chunks of 64 cells come from one atomic
+// counter per XCD, each over its own contiguous eighth of that list; a wave
+// whose eighth is used up moves on to the next one.  This keeps SIMD lanes
+// busy although searches need 1 to ~200 queries per azimuth, and keeps each
+// XCD's L2 working set spatially compact.
+// ---------------------------------------------------------------------------
+
+constexpr int kChunk = 64;
+enum { P_AZSTART = 0, P_FIRST = 1, P_UP = 2, P_DOWN = 3 };
+
+__global__ void k_clear_state(TraceState* ts) {
+    int* w = reinterpret_cast<int*>(ts);
+    for (int i = threadIdx.x; i < (int)(sizeof(TraceState) / 4); i += blockDim.x) w[i] = 0;
+}
+
+// Per-lane FP64 search state kept in global memory between queries (it is
+// needed only when a query ends), so that the traversal loop runs with the
+// registers of the traversal alone.  Field f of lane g is spill[f * lanes + g].
+enum { S_DIR = 0, S_AX = 3, S_E = 6, S_SECTOR = 7, S_SVF = 8, S_DK = 9, S_EK = 12, S_FIELDS = 13 };
+
+template <int MODE, bool COUNT, int BLOCK, int WAVES>
+__global__ __launch_bounds__(BLOCK, WAVES) void k_trace_pt(const BvhNode* __restrict__ nodes,
+                                                    const float4* __restrict__ tris, int num_tri,
+                                                    const CellFrame* __restrict__ cells,
+                                                    const int* __restrict__ order, WalkParams P,
+                                                    int refill, double* __restrict__ horizon,
+                                                    double* __restrict__ skyview,
+                                                    TraceState* __restrict__ ts,
+                                                    double* __restrict__ spill) {
+    __shared__ int stk_all[kLdsStack * BLOCK];
+    int* stk = stk_all + threadIdx.x;
+    int ovf[kStackMax - kLdsStack];
+    const int lane = threadIdx.x & 63;
+    const unsigned long long lt_mask = (1ull << lane) - 1ull;
+    const int N = P.ncell;
+    const int L = P.num_hori * P.refine_factor;
+    const double a = P.hori_acc;
+    const bool have = num_tri > 0;
+    int* queue = ts->queue;
+    const size_t lanes = (size_t)gridDim.x * BLOCK;
+    double* sv = spill + (size_t)blockIdx.x * BLOCK + threadIdx.x;
+#define SV(f) sv[(size_t)(f) * lanes]
+
+    // wave-uniform queue state
+    int cur = blockIdx.x & 7;  // workgroups are dealt round-robin over the 8 XCDs
+    int tries = 0;
+    int qb = 0, qe = 0;
+    bool exhausted = false;
+
+    // lane state kept in registers
+    bool has_cell = false, live = false, last_hit = false;
+    int cell = 0, phase = P_AZSTART, j = 0, ksec = 0, jsec = 0;
+    float o0 = 0.f, o1 = 0.f, o2 = 0.f;
+    Ray ray = make_ray(0.f, 0.f, 0.f, 1.f, 0.f, 0.f, P.tfar);
+    int node = kWideBit, sp = 0, steps = 0;
+    unsigned rays = 0, nvisit = 0, ntest = 0;
+    int err = 0;
+
+    while (true) {
+        // ---------------- refill ----------------
+        for (int pass = 0; pass < 4; pass++) {
+            const bool want = !live && !has_cell;
+            const unsigned long long need = __ballot(want);
+            if (need && !exhausted) {
+                if (qb >= qe) {
+                    while (tries < 8) {
+                        const int rlo = (int)(((long long)cur * N) >> 3);
+                        const int rhi = (int)(((long long)(cur + 1) * N) >> 3);
+                        int old = 0;
+                        if (lane == 0) old = atomicAdd(&queue[cur * kQueueStride], kChunk);
+                        old = __shfl(old, 0, 64);
+                        if (rlo + old < rhi) {
+                            qb = rlo + old;
+                            qe = min(qb + kChunk, rhi);
+                            break;
+                        }
+                        cur = (cur + 1) & 7;
+                        tries++;
+                    }
+                    if (tries >= 8) exhausted = true;
+                }
+                if (!exhausted) {
+                    const int avail = qe - qb;
+                    const int rank = __popcll(need & lt_mask);
+                    if (want && rank < avail) {
+                        cell = order ? order[qb + rank] : qb + rank;
+                        has_cell = true;
+                        const CellFrame f = cells[cell];
+                        const D3 n = {f.n[0], f.n[1], f.n[2]};
+                        const D3 north = {f.north[0], f.north[1], f.north[2]};
+                        o0 = f.org[0];
+                        o1 = f.org[1];
+                        o2 = f.org[2];
+                        const D3 dir = rodrigues(north, n, P.shift_sin, P.shift_cos);
+                        SV(S_DIR) = dir.x;
+                        SV(S_DIR + 1) = dir.y;
+                        SV(S_DIR + 2) = dir.z;
+                        SV(S_E) = 0.0;
+                        SV(S_SECTOR) = 0.0;
+                        SV(S_SVF) = 0.0;
+                        j = 0;
+                        ksec = 0;
+                        jsec = 0;
+                        phase = P_AZSTART;
+                    }
+                    qb += min(__popcll(need), avail);
+                }
+            }
+            if (has_cell && !live) {
+                // consume the last answer and issue the next query of this cell
+                const double* fn = cells[cell].n;
+                const D3 n = {fn[0], fn[1], fn[2]};
+                D3 dir = {SV(S_DIR), SV(S_DIR + 1), SV(S_DIR + 2)};
+                D3 ax = {SV(S_AX), SV(S_AX + 1), SV(S_AX + 2)};
+                double e = SV(S_E), sector = SV(S_SECTOR), svf = SV(S_SVF);
+                D3 dk = {0.0, 0.0, 0.0};
+                double ek = 0.0;
+                if (MODE == 1) {
+                    dk = {SV(S_DK), SV(S_DK + 1), SV(S_DK + 2)};
+                    ek = SV(S_EK);
+                }
+                D3 q = dir;  // direction of the query to issue
+                bool issue = false;
+                while (true) {
+                    if (phase == P_AZSTART) {
+                        ax = unit_cross(dir, n);
+                        phase = P_FIRST;
+                        q = dir;
+                        issue = true;
+                        break;
+                    }
+                    double h;
+                    if (MODE == 0) {  // reference walk (:521-546)
+                        if (phase == P_FIRST) {
+                            if (last_hit) {
+                                phase = P_UP;
+                                e += 2.0 * a;
+                                dir = rodrigues(dir, ax, P.elev_sin_2ha, P.elev_cos_2ha);
+                                q = dir;
+                                issue = true;
+                                break;
+                            }
+                            if (e > P.elev_thresh) {
+                                phase = P_DOWN;
+                                e -= 2.0 * a;
+                                dir = rodrigues(dir, ax, -P.elev_sin_2ha, P.elev_cos_2ha);
+                                q = dir;
+                                issue = true;
+                                break;
+                            }
+                            h = e + a;
+                        } else if (phase == P_UP) {
+                            if (last_hit) {
+                                if (!(e > kPi)) {
+                                    e += 2.0 * a;
+                                    dir = rodrigues(dir, ax, P.elev_sin_2ha, P.elev_cos_2ha);
+                                    q = dir;
+                                    issue = true;
+                                    break;
+                                }
+                                err |= 2;  // a ray past the zenith still hits: not a heightfield
+                            }
+                            h = e - a;
+                        } else {
+                            if (!last_hit && e > P.elev_thresh) {
+                                e -= 2.0 * a;
+                                dir = rodrigues(dir, ax, -P.elev_sin_2ha, P.elev_cos_2ha);
+                                q = dir;
+                                issue = true;
+                                break;
+                            }
+                            h = e + a;
+                        }
+                    } else {  // clip-aware climb from the horizontal direction
+                        if (phase == P_FIRST) {
+                            if (last_hit) {
+                                phase = P_UP;
+                                ek = 0.0;
+                                ek += 2.0 * a;
+                                dk = rodrigues(dir, ax, P.elev_sin_2ha, P.elev_cos_2ha);
+                                q = dk;
+                                issue = true;
+                                break;
+                            }
+                            h = -a;
+                        } else {
+                            if (last_hit) {
+                                if (!(ek > kPi)) {
+                                    ek += 2.0 * a;
+                                    dk = rodrigues(dk, ax, P.elev_sin_2ha, P.elev_cos_2ha);
+                                    q = dk;
+                                    issue = true;
+                                    break;
+                                }
+                                err |= 2;
+                            }
+                            h = ek - a;
+                        }
+                    }
+                    // azimuth finished: clip, sector mean, SVF (:736-754)
+                    if (h < 0.0) h = 0.0;
+                    sector += h;
+                    if (++ksec == P.refine_factor) {
+                        horizon[(size_t)jsec * N + cell] =
+                            ((sector / kPi) * 180.0) / (double)P.refine_factor;
+                        sector = 0.0;
+                        ksec = 0;
+                        jsec++;
+                    }
+                    const double sn = sin(h);
+                    if (P.svf_type == 0)
+                        svf += (1.0 - sn);
+                    else if (P.svf_type == 1)
+                        svf += (1.0 - (sn * sn));
+                    else
+                        svf += (1.0 - (sn * sn * sn));
+                    dir = rodrigues(dir, n, -P.azim_sin, P.azim_cos);  // (:550-551)
+                    if (++j == L) {
+                        skyview[cell] = svf / (double)L;
+                        has_cell = false;
+                        break;
+                    }
+                    phase = P_AZSTART;
+                }
+                if (issue) {
+                    rays++;
+                    ray = make_ray(o0, o1, o2, (float)q.x, (float)q.y, (float)q.z, P.tfar);
+                    node = kWideBit;  // the root is read in full
+                    sp = 0;
+                    steps = 0;
+                    live = true;
+                    SV(S_DIR) = dir.x;
+                    SV(S_DIR + 1) = dir.y;
+                    SV(S_DIR + 2) = dir.z;
+                    SV(S_AX) = ax.x;
+                    SV(S_AX + 1) = ax.y;
+                    SV(S_AX + 2) = ax.z;
+                    SV(S_E) = e;
+                    SV(S_SECTOR) = sector;
+                    SV(S_SVF) = svf;
+                    if (MODE == 1) {
+                        SV(S_DK) = dk.x;
+                        SV(S_DK + 1) = dk.y;
+                        SV(S_DK + 2) = dk.z;
+                        SV(S_EK) = ek;
+                    }
+                }
+            }
+            if (!__any(!live && !has_cell && !exhausted)) break;
+        }
+        // keep the search state in memory, not in registers, across the traversal
+        __asm__ __volatile__("" ::: "memory");
+        if (!__any(live)) break;
+        // ---------------- traversal ----------------
+        const int thresh = exhausted ? 1 : refill;
+        while (true) {
+            if (live) {
+                int s = 2;
+                if (have) {
+                    s = trav_step<COUNT, BLOCK>(nodes, tris, stk, ovf, ray, node, sp, nvisit, ntest,
+                                                err);
+                    // a BVH of n triangles has n - 1 interior nodes; more steps than
+                    // that in one query can only come from a malformed tree
+                    if (s == 0 && ++steps > (1 << 28)) {
+                        err |= 4;
+                        s = 1;
+                    }
+                }
+                if (s) {
+                    live = false;
+                    last_hit = (s == 1);
+                }
+            }
+            if (__popcll(__ballot(live)) < thresh) break;
+        }
+    }
+#undef SV
+    const unsigned long long wr = wave_sum(rays);
+    const unsigned long long wn = COUNT ? wave_sum(nvisit) : 0ull;
+    const unsigned long long wt = COUNT ? wave_sum(ntest) : 0ull;
+    const int werr = __any(err != 0);
+    if (lane == 0) {
+        if (wr) atomicAdd(&ts->cnt.rays, wr);
+        if (COUNT) {
+            atomicAdd(&ts->cnt.nodes, wn);
+            atomicAdd(&ts->cnt.tris, wt);
+        }
+        if (werr) atomicOr(&ts->cnt.error, 1ull);
+    }
+}
+
+template <int MODE, bool COUNT, int BLOCK, int WAVES>
+int pt_grid(int ncell) {
+    static const int occ_grid = []() {
+        int dev = 0, cus = 0, per = 0;
+        if (hipGetDevice(&dev) != hipSuccess) return 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+            return 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &per, reinterpret_cast<const void*>(&k_trace_pt<MODE, COUNT, BLOCK, WAVES>), BLOCK,
+                0) !=
+            hipSuccess)
+            return 0;
+        return std::max(1, per) * std::max(1, cus);
+    }();
+    int want = (ncell + BLOCK - 1) / BLOCK;
+    int g = occ_grid > 0 ? std::min(occ_grid, want) : want;
+    return std::max(8, (g + 7) / 8 * 8);
+}
+
+template <int MODE, bool COUNT, int BLOCK, int WAVES>
+void launch_pt(const DeviceBvh& bvh, const CellFrame* cells, const int* order,
+               const WalkParams& p, int refill, double* horizon, double* skyview,
+               TraceState* ts, double* spill, hipStream_t st) {
+    hipLaunchKernelGGL((k_trace_pt<MODE, COUNT, BLOCK, WAVES>),
+                       dim3(pt_grid<MODE, COUNT, BLOCK, WAVES>(p.ncell)), dim3(BLOCK), 0, st,
+                       bvh.nodes, bvh.tris, bvh.num_tri, cells, order, p, refill, horizon,
+                       skyview, ts, spill);
+}
+
 template <int MODE, bool COUNT, int BLOCK>
 void launch_one(const DeviceBvh& bvh, const CellFrame* cells, const int* order,
-                const WalkParams& p, double* horizon, double* skyview, TraceCounters* cnt,
-                hipStream_t st) {
+                const WalkParams& p, int kernel, int refill, double* horizon, double* skyview,
+                TraceState* ts, double* spill, hipStream_t st) {
+    if (kernel == 0) {
+        launch_pt<MODE, COUNT, BLOCK, 1>(bvh, cells, order, p, refill, horizon, skyview, ts,
+                                         spill, st);
+        return;
+    }
+    if (kernel == 2 || kernel == 3) {  // persistent, occupancy target 4 / 5 waves per SIMD
+        if (kernel == 2)
+            launch_pt<MODE, COUNT, BLOCK, 4>(bvh, cells, order, p, refill, horizon, skyview, ts,
+                                             spill, st);
+        else
+            launch_pt<MODE, COUNT, BLOCK, 5>(bvh, cells, order, p, refill, horizon, skyview, ts,
+                                             spill, st);
+        return;
+    }
+    TraceCounters* cnt = &ts->cnt;
     int nb = (p.ncell + BLOCK - 1) / BLOCK;
     int g = (nb + 7) / 8 * 8;
     hipLaunchKernelGGL((k_trace<MODE, COUNT, BLOCK>), dim3(g), dim3(BLOCK), 0, st, bvh.nodes,
@@ -360,31 +737,51 @@ void launch_one(const DeviceBvh& bvh, const CellFrame* cells, const int* order,
 
 template <int MODE, bool COUNT>
 void launch_block(int block, const DeviceBvh& bvh, const CellFrame* cells, const int* order,
-                  const WalkParams& p, double* horizon, double* skyview, TraceCounters* cnt,
-                  hipStream_t st) {
+                  const WalkParams& p, int kernel, int refill, double* horizon, double* skyview,
+                  TraceState* ts, double* spill, hipStream_t st) {
     if (block == 64)
-        launch_one<MODE, COUNT, 64>(bvh, cells, order, p, horizon, skyview, cnt, st);
+        launch_one<MODE, COUNT, 64>(bvh, cells, order, p, kernel, refill, horizon, skyview, ts,
+                                    spill, st);
     else if (block == 256)
-        launch_one<MODE, COUNT, 256>(bvh, cells, order, p, horizon, skyview, cnt, st);
+        launch_one<MODE, COUNT, 256>(bvh, cells, order, p, kernel, refill, horizon, skyview, ts,
+                                     spill, st);
     else
-        launch_one<MODE, COUNT, 128>(bvh, cells, order, p, horizon, skyview, cnt, st);
+        launch_one<MODE, COUNT, 128>(bvh, cells, order, p, kernel, refill, horizon, skyview, ts,
+                                     spill, st);
 }
 
 }  // namespace
 
+size_t trace_spill_doubles(int ncell) {
+    // the persistent grid never holds more lanes than the device can keep
+    // resident (32 waves of 64 lanes per CU) nor more than ncell + 8 blocks
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess)
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    size_t lanes = std::min((size_t)cus * 2048, (size_t)ncell + 8 * 256);
+    return lanes * S_FIELDS;
+}
+
 int launch_trace(const DeviceBvh& bvh, const CellFrame* cells, const int* order,
-                 const WalkParams& p, int mode, int block, bool counters, double* horizon,
-                 double* skyview, TraceCounters* cnt, hipStream_t st, std::string& err) {
+                 const WalkParams& p, int mode, int kernel, int block, int refill, bool counters,
+                 double* horizon, double* skyview, TraceState* ts, double* spill, hipStream_t st,
+                 std::string& err) {
+    if (refill < 1 || refill > 64) refill = 40;
+    hipLaunchKernelGGL(k_clear_state, dim3(1), dim3(256), 0, st, ts);
     if (mode == 0) {
         if (counters)
-            launch_block<0, true>(block, bvh, cells, order, p, horizon, skyview, cnt, st);
+            launch_block<0, true>(block, bvh, cells, order, p, kernel, refill, horizon, skyview, ts,
+                                  spill, st);
         else
-            launch_block<0, false>(block, bvh, cells, order, p, horizon, skyview, cnt, st);
+            launch_block<0, false>(block, bvh, cells, order, p, kernel, refill, horizon, skyview, ts,
+                                   spill, st);
     } else {
         if (counters)
-            launch_block<1, true>(block, bvh, cells, order, p, horizon, skyview, cnt, st);
+            launch_block<1, true>(block, bvh, cells, order, p, kernel, refill, horizon, skyview, ts,
+                                  spill, st);
         else
-            launch_block<1, false>(block, bvh, cells, order, p, horizon, skyview, cnt, st);
+            launch_block<1, false>(block, bvh, cells, order, p, kernel, refill, horizon, skyview, ts,
+                                   spill, st);
     }
     HZ_HIP(hipGetLastError(), err);
     return 0;

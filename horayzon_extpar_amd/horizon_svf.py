@@ -12,7 +12,8 @@ Differences, all of them supersets:
 * a failing device call raises ``HzError`` (RuntimeError) instead of
   printing and returning garbage;
 * keyword-only ``search_mode``, ``leaf_size``, ``block_size``,
-  ``collect_counters`` and ``stats`` expose the library options.
+  ``collect_counters``, ``trace_kernel``, ``refill`` and ``stats`` expose
+  the library options.
 """
 
 from __future__ import annotations
@@ -99,7 +100,7 @@ def validate(clon, clat, hsurf, vlon, vlat, cells_of_vertex, num_hori, grid_type
 def horizon_svf_comp_py(clon, clat, hsurf, vlon, vlat, cells_of_vertex, num_hori,
                         grid_type, dist_search, ray_org_elev, refine_factor, svf_type, *,
                         search_mode=None, leaf_size=None, block_size=None,
-                        collect_counters=None, stats=None):
+                        collect_counters=None, trace_kernel=None, refill=None, stats=None):
     """Compute the terrain horizon [deg] and sky view factor [-].
 
     Same contract as the reference binding (horizon_svf.pyx:15-124):
@@ -117,7 +118,8 @@ def horizon_svf_comp_py(clon, clat, hsurf, vlon, vlat, cells_of_vertex, num_hori
     clon, clat, hsurf, vlon, vlat = (np.ascontiguousarray(x) for x in
                                      (clon, clat, hsurf, vlon, vlat))
     opt = default_options(search_mode=search_mode, leaf_size=leaf_size,
-                          block_size=block_size, collect_counters=collect_counters)
+                          block_size=block_size, collect_counters=collect_counters,
+                          trace_kernel=trace_kernel, refill=refill)
     st = HzStats()
     rc = _lib.lib().horizon_svf_comp_ex(
         _ptr(clon, ctypes.c_double), _ptr(clat, ctypes.c_double),

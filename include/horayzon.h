@@ -36,7 +36,12 @@ typedef struct hz_options {
     int collect_counters; /* 1: count BVH node visits and triangle tests (slower) */
     int cell_order;       /* 1: trace cells in Morton order of their position (default), 0: input order */
     int verbose;          /* 1: print the reference's progress lines to stderr */
-    int reserved[10];
+    int trace_kernel;     /* 0: persistent lanes with per-XCD work queues (default),
+                             1: one lane per cell with azimuths in lock step,
+                             2 / 3: persistent, compiled for 4 / 5 waves per SIMD */
+    int refill;           /* persistent kernel: refill lanes once fewer than this many of
+                             the 64 lanes of a wavefront hold a live query (1..64; 0 = default) */
+    int reserved[8];
 } hz_options;
 
 typedef struct hz_stats {

@@ -56,6 +56,18 @@ def test_c1_climb_mode(c1):
     assert np.max(np.abs(s - so)) <= SVF_TOL
 
 
+@pytest.mark.parametrize("kernel,block,refill",
+                         [(1, 128, 0), (0, 64, 64), (0, 256, 8), (2, 128, 0), (3, 128, 0)])
+def test_kernel_variants_bit_exact(c1, kernel, block, refill):
+    print(f"case c1 kernel={kernel} block={block} refill={refill}", flush=True)
+    h, s = horizon_svf_comp_py(*c1.args(), 24, 1, 40000.0, 0.2, 1, 2, trace_kernel=kernel,
+                               block_size=block, refill=refill or None)
+    ho, so, _ = oracle.horizon_svf(c1, grid_type=1)
+    _report(f"c1 kernel{kernel}", h, ho)
+    assert np.array_equal(h, ho)
+    assert np.max(np.abs(s - so)) <= SVF_TOL
+
+
 @pytest.mark.parametrize("leaf", [1, 8])
 def test_leaf_size_independent(c1, leaf):
     # the occlusion answer is the OR over all triangles: any tree gives the same bits
