@@ -40,6 +40,8 @@ def main():
     ap.add_argument("--kernel", type=int, default=0, help="0 persistent, 1 one lane per cell")
     ap.add_argument("--refill", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--weights", default="")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--cpu-threads", type=int, default=0)
     args = ap.parse_args()
@@ -78,6 +80,9 @@ def main():
             rays = st["num_rays"]
         t1 = time.perf_counter()
         h, s = dom.fetch()
+        learned_out = None
+        if not args.no_learned and mode == 0:
+            learned_out = learned_leg(dom, args, h, s, dom_in, (t1 - t0) / args.steps)
     ncell = dom_in.ncell
     ms_step = (t1 - t0) * 1e3 / args.steps
     value = ncell / (ms_step * 1e-3)
@@ -116,9 +121,76 @@ def main():
                      "nodes_per_ray": n_node, "tris_per_ray": n_tri},
         "horizon_checksum": float(np.sum(h)), "skyview_mean": float(np.mean(s)),
     }
+    if learned_out:
+        out["learned"] = learned_out
+        out["learned_vs_plain_speedup"] = learned_out["speedup_vs_plain"]
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(dom_in, args.cpu_budget, args.cpu_threads)
     print(json.dumps(out), flush=True)
+
+
+def learned_leg(dom, args, h_walk, s_walk, dom_in, plain_step_s):
+    """Learned-bracket form on the same resident domain: each step runs the
+    LBVH build, cell ordering, raster + descriptors + predictor and the
+    lattice search started at the prediction."""
+    import numpy as np
+    from horayzon_extpar_amd import learned
+
+    path = args.weights or learned.DEFAULT_WEIGHTS
+    if not os.path.exists(path):
+        print(f"[bench] no predictor weights at {path}; learned leg skipped", file=sys.stderr)
+        return None
+    pred = learned.Predictor.from_file(path)
+    dom.set_predictor(pred)
+    opts = dict(search_mode=2)
+    for _ in range(max(1, args.warmup)):
+        dom.run(24, 40000.0, 1, 2, **opts)
+    tp, tt = [], []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        st = dom.run(24, 40000.0, 1, 2, **opts)
+        tp.append(st["t_predict_ms"])
+        tt.append(st["t_trace_ms"])
+    t1 = time.perf_counter()
+    h, s = dom.fetch()
+    P = dom.predictions()
+    step = (t1 - t0) / args.steps
+    diff = np.abs(h - h_walk) > 1e-9
+    cells_diff = np.any(diff, axis=0)
+    border = border_affected(dom_in, 24, 40000.0)
+    err = np.abs(P - h_walk.T)
+    dom.set_predictor(None)
+    return {"value": dom_in.ncell / step, "unit": "cells/s", "ms_per_step": step * 1e3,
+            "ms_predict": float(np.mean(tp)), "ms_trace": float(np.mean(tt)),
+            "rays": st["num_rays"], "rays_per_cell_sector": st["num_rays"] / (dom_in.ncell * 24),
+            "speedup_vs_plain": plain_step_s / step,
+            "angles_differing_from_walk": int(diff.sum()),
+            "cells_differing_from_walk": int(cells_diff.sum()),
+            "angles_differing_off_border": int((diff & ~border).sum()),
+            "angles_border_affected": int(border.sum()),
+            "max_abs_dh_deg": float(np.abs(h - h_walk).max()),
+            "max_abs_dsvf": float(np.abs(s - s_walk).max()),
+            "prediction_mae_deg": float(err.mean()),
+            "weights": os.path.relpath(path, ROOT)}
+
+
+def border_affected(dom_in, num_hori, dist_search, margin_m=3000.0):
+    """[num_hori, ncell] mask of azimuths that can meet the holes of the
+    grid_type-1 mesh at the lattice border (dropped border vertices), where
+    hits need not be monotone in elevation and the reference walk is
+    path-dependent: the cell lies within margin_m of the border, or the
+    azimuth's ground track within dist_search comes that close to it."""
+    import numpy as np
+    hw = dom_in.meta["width_km"] * 500.0 - margin_m
+    hh = dom_in.meta["height_km"] * 500.0 - margin_m
+    phi = np.arange(num_hori) * (2 * np.pi / num_hori)
+    dx, dy = np.sin(phi)[:, None], np.cos(phi)[:, None]
+    x, y = dom_in.cx[None, :], dom_in.cy[None, :]
+    with np.errstate(divide="ignore", invalid="ignore"):
+        tx = np.where(dx > 0, (hw - x) / dx, np.where(dx < 0, (-hw - x) / dx, np.inf))
+        ty = np.where(dy > 0, (hh - y) / dy, np.where(dy < 0, (-hh - y) / dy, np.inf))
+    near = (np.abs(x) > hw) | (np.abs(y) > hh)
+    return (np.minimum(tx, ty) < dist_search * 1.02) | near
 
 
 def cpu_baseline(dom_in, budget_s, threads):

@@ -24,6 +24,7 @@ HZ_EKERNEL = -4
 
 HZ_MODE_WALK = 0
 HZ_MODE_CLIMB = 1
+HZ_MODE_LEARNED = 2
 
 
 class HzOptions(ctypes.Structure):
@@ -43,7 +44,8 @@ class HzStats(ctypes.Structure):
                 ("t_bvh_ms", ctypes.c_double), ("t_order_ms", ctypes.c_double),
                 ("t_trace_ms", ctypes.c_double), ("t_download_ms", ctypes.c_double),
                 ("t_total_ms", ctypes.c_double), ("lon0", ctypes.c_double),
-                ("lat0", ctypes.c_double), ("reserved", ctypes.c_double * 8)]
+                ("lat0", ctypes.c_double), ("t_predict_ms", ctypes.c_double),
+                ("reserved", ctypes.c_double * 7)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
@@ -77,6 +79,29 @@ SIGNATURES = [
               ctypes.POINTER(ctypes.c_float), c_int_p, c_double_p, c_double_p,
               ctypes.POINTER(ctypes.c_float), c_double_p]),
     ("hz_domain_num_cell", ctypes.c_int, [ctypes.c_void_p]),
+    ("hz_predictor_num_params", ctypes.c_longlong, []),
+    ("hz_predictor_create", ctypes.c_int,
+     [ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_void_p)]),
+    ("hz_predictor_destroy", None, [ctypes.c_void_p]),
+    ("hz_domain_set_predictor", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    ("hz_domain_descriptors", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint16)]),
+    ("hz_domain_predictions", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)]),
+    ("hz_trainer_create", ctypes.c_int, [ctypes.c_ulonglong, ctypes.POINTER(ctypes.c_void_p)]),
+    ("hz_trainer_destroy", None, [ctypes.c_void_p]),
+    ("hz_trainer_set_weights", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)]),
+    ("hz_trainer_get_weights", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)]),
+    ("hz_trainer_set_data", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint16), ctypes.POINTER(ctypes.c_float),
+      ctypes.c_int]),
+    ("hz_trainer_fit", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_float,
+      ctypes.POINTER(ctypes.c_float)]),
+    ("hz_trainer_gradients", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint16), ctypes.POINTER(ctypes.c_float),
+      ctypes.c_int, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]),
+    ("hz_trainer_predict", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint16), ctypes.c_int,
+      ctypes.POINTER(ctypes.c_float)]),
     ("hz_domain_num_triangles", ctypes.c_longlong, [ctypes.c_void_p]),
 ]
 

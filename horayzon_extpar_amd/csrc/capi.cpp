@@ -20,6 +20,8 @@
 
 #include "../../include/horayzon.h"
 #include "hz_internal.h"
+#include "learn_internal.h"
+#include "mlp.h"
 
 using namespace hz;
 
@@ -80,6 +82,10 @@ int check_run_args(int num_hori, double dist_search, int refine_factor, int svf_
 
 }  // namespace
 
+namespace hz {
+int learn_fail(int code, const std::string& msg) { return fail(code, msg); }
+}  // namespace hz
+
 struct hz_domain {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -100,9 +106,18 @@ struct hz_domain {
     int last_nhori = 0; // rows written by the last run
     TraceState* d_state = nullptr;
     double* d_spill = nullptr;
+    double2* d_elev_cs = nullptr;
+    float bounds_xy[4] = {0, 0, 0, 0};  // scene x/y extent (raster of the descriptor)
+    // learned mode
+    const hz_predictor* predictor = nullptr;
+    Raster raster;
+    uint16_t* d_desc = nullptr;
+    float* d_pred = nullptr;
+    MlpWork mlpw;
+    bool have_pred = false;
     LbvhWork work;
     DeviceBvh bvh;
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     bool ran = false;
 };
 
@@ -113,7 +128,7 @@ const char* hz_last_error(void) { return g_err.c_str(); }
 void hz_default_options(hz_options* o) {
     std::memset(o, 0, sizeof(*o));
     o->search_mode = HZ_MODE_WALK;
-    o->leaf_size = 4;
+    o->leaf_size = 1;
     o->block_size = 128;
     o->collect_counters = 0;
     o->cell_order = 1;
@@ -128,10 +143,12 @@ void hz_domain_destroy(hz_domain* d) {
     if (d->stream) (void)hipStreamSynchronize(d->stream);
     void* bufs[] = {d->d_vert,   d->d_tri,     d->d_cells,   d->d_order, d->d_nodes,
                     d->d_tris,   d->d_horizon, d->d_skyview, d->d_state,
-                    d->d_spill};
+                    d->d_spill,  d->d_elev_cs, d->d_desc,    d->d_pred};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     lbvh_free(d->work);
+    d->raster.free_all();
+    d->mlpw.free_all();
     for (auto& e : d->ev)
         if (e) (void)hipEventDestroy(e);
     if (d->stream) (void)hipStreamDestroy(d->stream);
@@ -232,6 +249,24 @@ int hz_domain_create(const double* clon, const double* clat, const double* hsurf
     DCHK(hipMalloc(&d->d_skyview, (size_t)num_cell * sizeof(double)));
     DCHK(hipMalloc(&d->d_state, sizeof(TraceState)));
     DCHK(hipMalloc(&d->d_spill, trace_spill_doubles(num_cell) * sizeof(double)));
+    DCHK(hipMalloc(&d->d_elev_cs, (kMaxLattice + 1) * (sizeof(double2) + sizeof(double))));
+    {
+        std::vector<double2> cs(kMaxLattice + 1);
+        std::vector<double> sh(kMaxLattice + 1);
+        lattice_table(make_walk_params(24, 1, 2, 4e4, num_cell).hori_acc, cs.data(), sh.data());
+        DCHK(hipMemcpy(d->d_elev_cs, cs.data(), cs.size() * sizeof(double2), hipMemcpyHostToDevice));
+        DCHK(hipMemcpy(d->d_elev_cs + cs.size(), sh.data(), sh.size() * sizeof(double),
+                       hipMemcpyHostToDevice));
+        float b[4] = {3e38f, 3e38f, -3e38f, -3e38f};
+        for (size_t i = 0; i < nv; i += 3) {
+            b[0] = std::min(b[0], hs.vert32[i]);
+            b[1] = std::min(b[1], hs.vert32[i + 1]);
+            b[2] = std::max(b[2], hs.vert32[i]);
+            b[3] = std::max(b[3], hs.vert32[i + 1]);
+        }
+        if (nv == 0) b[0] = b[1] = b[2] = b[3] = 0.0f;
+        std::memcpy(d->bounds_xy, b, sizeof(b));
+    }
     if (nv) DCHK(hipMemcpyAsync(d->d_vert, hs.vert32.data(), nv * sizeof(float), hipMemcpyHostToDevice, d->stream));
     if (nt) DCHK(hipMemcpyAsync(d->d_tri, hs.tri.data(), nt * sizeof(int), hipMemcpyHostToDevice, d->stream));
     DCHK(hipMemcpyAsync(d->d_cells, hs.frame.data(), (size_t)num_cell * sizeof(CellFrame),
@@ -264,6 +299,77 @@ int hz_domain_create(const double* clon, const double* clat, const double* hsurf
                         std::string(#call) + ": " + hipGetErrorString(e_));            \
     } while (0)
 
+// Raster, descriptors and predictor on all cells of d (enqueued on d's stream).
+static int predict_cells(hz_domain* d, std::string& err) {
+    const int n = d->ncell;
+    if (!d->d_desc) {
+        if (hipMalloc(&d->d_desc, (size_t)n * kIn * sizeof(uint16_t)) != hipSuccess ||
+            hipMalloc(&d->d_pred, (size_t)n * kOutPad * sizeof(float)) != hipSuccess) {
+            err = "hipMalloc failed (descriptors)";
+            return HZ_ENOMEM;
+        }
+    }
+    const float* b = d->bounds_xy;
+    double area = std::max(1.0, (double)(b[2] - b[0]) * (double)(b[3] - b[1]));
+    float pix = (float)std::sqrt(area / std::max<long long>(1, d->ntri));
+    if (raster_build(d->raster, d->d_vert, d->d_tri, (int)d->ntri, b, pix, d->stream, err))
+        return HZ_EDEVICE;
+    descriptors(d->raster, d->d_cells, n, d->d_desc, d->stream);
+    if (d->predictor) {
+        const int chunk = 1 << 16;
+        if (d->mlpw.reserve(std::min(n, chunk), err)) return HZ_ENOMEM;
+        for (int s = 0; s < n; s += chunk) {
+            int m = std::min(chunk, n - s);
+            mlp_infer(d->predictor->net, d->d_desc + (size_t)s * kIn, m, d->mlpw,
+                      d->d_pred + (size_t)s * kOutPad, d->stream);
+        }
+        d->have_pred = true;
+    }
+    if (hipGetLastError() != hipSuccess) {
+        err = "descriptor / predictor launch failed";
+        return HZ_EDEVICE;
+    }
+    return 0;
+}
+
+int hz_domain_set_predictor(hz_domain* d, const hz_predictor* p) {
+    g_err.clear();
+    if (!d) return fail(HZ_EINVAL, "domain is NULL");
+    d->predictor = p;
+    d->have_pred = false;
+    return HZ_OK;
+}
+
+int hz_domain_descriptors(hz_domain* d, unsigned short* desc) {
+    g_err.clear();
+    if (!d || !desc) return fail(HZ_EINVAL, "NULL argument");
+    RCHK(hipSetDevice(d->device));
+    std::string err;
+    const hz_predictor* keep = d->predictor;
+    d->predictor = nullptr;  // descriptors only
+    int rc = predict_cells(d, err);
+    d->predictor = keep;
+    if (rc) return fail(rc, err);
+    RCHK(hipMemcpyAsync(desc, d->d_desc, (size_t)d->ncell * kIn * sizeof(uint16_t),
+                        hipMemcpyDeviceToHost, d->stream));
+    RCHK(hipStreamSynchronize(d->stream));
+    return HZ_OK;
+}
+
+int hz_domain_predictions(hz_domain* d, float* pred) {
+    g_err.clear();
+    if (!d || !pred) return fail(HZ_EINVAL, "NULL argument");
+    if (!d->have_pred) return fail(HZ_EINVAL, "no learned run to read predictions from");
+    RCHK(hipSetDevice(d->device));
+    std::vector<float> tmp((size_t)d->ncell * kOutPad);
+    RCHK(hipMemcpyAsync(tmp.data(), d->d_pred, tmp.size() * sizeof(float), hipMemcpyDeviceToHost,
+                        d->stream));
+    RCHK(hipStreamSynchronize(d->stream));
+    for (int i = 0; i < d->ncell; i++)
+        std::memcpy(pred + (size_t)i * kOut, tmp.data() + (size_t)i * kOutPad, kOut * sizeof(float));
+    return HZ_OK;
+}
+
 int hz_domain_run(hz_domain* d, int num_hori, double dist_search, int refine_factor,
                   int svf_type, const hz_options* opt, hz_stats* stats) {
     g_err.clear();
@@ -275,12 +381,20 @@ int hz_domain_run(hz_domain* d, int num_hori, double dist_search, int refine_fac
         o = *opt;
     else
         hz_default_options(&o);
-    if (o.search_mode != HZ_MODE_WALK && o.search_mode != HZ_MODE_CLIMB)
+    if (o.search_mode != HZ_MODE_WALK && o.search_mode != HZ_MODE_CLIMB &&
+        o.search_mode != HZ_MODE_LEARNED)
         return fail(HZ_EINVAL, "unknown search_mode");
+    if (o.search_mode == HZ_MODE_LEARNED) {
+        if (!d->predictor) return fail(HZ_EINVAL, "learned search: no predictor set");
+        if (num_hori * refine_factor != kDescAz)
+            return fail(HZ_EINVAL, "learned search needs num_hori * refine_factor == 24");
+        if (o.trace_kernel == 1)
+            return fail(HZ_EINVAL, "learned search needs a persistent trace kernel");
+    }
     int block = o.block_size ? o.block_size : 128;
     if (block != 64 && block != 128 && block != 256)
         return fail(HZ_EINVAL, "block_size must be 64, 128 or 256");
-    int leaf = o.leaf_size ? o.leaf_size : 4;
+    int leaf = o.leaf_size ? o.leaf_size : 1;
     if (leaf < 1 || leaf > kMaxLeafSize) return fail(HZ_EINVAL, "leaf_size must be in [1, 8]");
     bool verbose = o.verbose || verbose_env();
     RCHK(hipSetDevice(d->device));
@@ -308,8 +422,17 @@ int hz_domain_run(hz_domain* d, int num_hori, double dist_search, int refine_fac
     }
     RCHK(hipEventRecord(d->ev[2], st));
     WalkParams p = make_walk_params(num_hori, refine_factor, svf_type, dist_search, d->ncell);
-    if (o.trace_kernel < 0 || o.trace_kernel > 3)
-        return fail(HZ_EINVAL, "trace_kernel must be in [0, 3]");
+    p.elev_cs = d->d_elev_cs;
+    p.sin_h = reinterpret_cast<const double*>(d->d_elev_cs + (kMaxLattice + 1));
+    if (o.search_mode == HZ_MODE_LEARNED) {
+        rc = predict_cells(d, err);
+        if (rc) return fail(rc, err);
+        p.pred = d->d_pred;
+        p.pred_stride = kOutPad;
+    }
+    RCHK(hipEventRecord(d->ev[4], st));
+    if (o.trace_kernel < 0 || o.trace_kernel > 1)
+        return fail(HZ_EINVAL, "trace_kernel must be 0 or 1");
     int refill = o.refill ? o.refill : 40;
     if (refill < 1 || refill > 64) return fail(HZ_EINVAL, "refill must be in [1, 64]");
     rc = launch_trace(d->bvh, d->d_cells, order, p, o.search_mode, o.trace_kernel, block, refill,
@@ -323,7 +446,9 @@ int hz_domain_run(hz_domain* d, int num_hori, double dist_search, int refine_fac
     float ms_bvh = 0, ms_ord = 0, ms_trace = 0;
     RCHK(hipEventElapsedTime(&ms_bvh, d->ev[0], d->ev[1]));
     RCHK(hipEventElapsedTime(&ms_ord, d->ev[1], d->ev[2]));
-    RCHK(hipEventElapsedTime(&ms_trace, d->ev[2], d->ev[3]));
+    float ms_pred = 0;
+    RCHK(hipEventElapsedTime(&ms_pred, d->ev[2], d->ev[4]));
+    RCHK(hipEventElapsedTime(&ms_trace, d->ev[4], d->ev[3]));
     double t1 = now_ms();
     d->last_nhori = num_hori;
     d->ran = c.error == 0;
@@ -346,6 +471,7 @@ int hz_domain_run(hz_domain* d, int num_hori, double dist_search, int refine_fac
         stats->t_bvh_ms = ms_bvh;
         stats->t_order_ms = ms_ord;
         stats->t_trace_ms = ms_trace;
+        stats->t_predict_ms = ms_pred;
         stats->t_total_ms = t1 - t0;
         stats->lon0 = d->lon0;
         stats->lat0 = d->lat0;

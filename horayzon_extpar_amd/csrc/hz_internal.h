@@ -56,6 +56,7 @@ struct DeviceBvh {
     int num_tri = 0;
     int num_nodes = 0;  // interior nodes (>= 1 whenever num_tri >= 1)
     int depth = 0;      // deepest interior level, root = 0
+    int leaf_size = 1;  // largest leaf (> 1: leaves beyond the inline triangles exist)
 };
 
 // Constants of the elevation search (reference :577-587, :651-668).
@@ -70,7 +71,22 @@ struct WalkParams {
     int refine_factor;
     int svf_type;
     int ncell;
+    // lattice search (climb / learned modes): cos, sin of k * 0.5 deg for
+    // k = 0 .. kMaxLattice, and the predicted horizon [deg] per cell
+    // (pred[cell * pred_stride + azimuth]) of the learned mode
+    const double2* elev_cs;
+    const float* pred;
+    int pred_stride;
+    // sin((2k + 1) * hori_acc), k = 0 .. kMaxLattice: every horizon value is
+    // (2k + 1) * hori_acc (to the walk's accumulated rounding) or 0
+    const double* sin_h;
 };
+
+constexpr int kMaxLattice = 180;  // 90 deg
+
+// cos / sin of k * (2 * hori_acc), k = 0 .. kMaxLattice, through glibc
+// sincos, and sin((2k + 1) * hori_acc) through glibc sin (host)
+void lattice_table(double hori_acc, double2* out, double* sin_h);
 
 // ---------------------------------------------------------------------------
 // Host prologue (prologue.cpp)

@@ -420,6 +420,7 @@ int lbvh_build(LbvhWork& w, const float* vert, const int* tri, int n, int leaf_s
     out.num_tri = n;
     out.num_nodes = 0;
     out.depth = 0;
+    out.leaf_size = std::max(1, std::min(leaf_size, kMaxLeafSize));
     if (n == 0) return 0;
     if (lbvh_reserve(w, n, err)) return -3;
     if (leaf_size < 1) leaf_size = 1;

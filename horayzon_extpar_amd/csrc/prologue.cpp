@@ -89,6 +89,16 @@ inline int sorted_cells(const double* clon, const double* clat, const double* vl
 
 }  // namespace
 
+void lattice_table(double hori_acc, double2* out, double* sin_h) {
+    for (int k = 0; k <= kMaxLattice; k++) {
+        double sn, cs;
+        sin_cos((double)k * (2.0 * hori_acc), &sn, &cs);
+        out[k].x = cs;
+        out[k].y = sn;
+        sin_h[k] = std::sin((double)(2 * k + 1) * hori_acc);
+    }
+}
+
 WalkParams make_walk_params(int num_hori, int refine_factor, int svf_type, double dist_search,
                             int ncell) {
     WalkParams p;
@@ -106,6 +116,10 @@ WalkParams make_walk_params(int num_hori, int refine_factor, int svf_type, doubl
     p.refine_factor = refine_factor;
     p.svf_type = svf_type;
     p.ncell = ncell;
+    p.elev_cs = nullptr;
+    p.pred = nullptr;
+    p.pred_stride = 0;
+    p.sin_h = nullptr;
     return p;
 }
 

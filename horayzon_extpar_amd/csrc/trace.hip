@@ -53,6 +53,32 @@ __device__ __forceinline__ D3 unit_cross(const D3& a, const D3& b) {
     return c;
 }
 
+// Direction at lattice elevation k of the horizontal azimuth direction dir:
+// cos(k * 0.5 deg) dir + sin(k * 0.5 deg) n (cos / sin from the host table).
+__device__ __forceinline__ D3 lattice_dir(const D3& dir, const D3& n, const double2* cs, int k) {
+    const double2 t = cs[k];
+    return {t.x * dir.x + t.y * n.x, t.x * dir.y + t.y * n.y, t.x * dir.z + t.y * n.z};
+}
+
+// Lattice index nearest to a predicted horizon p [deg]: the search that
+// starts there needs two queries when p is within 0.25 deg of the result.
+__device__ __forceinline__ int start_index(float p) {
+    float kf = floorf(p * 2.0f + 0.5f);
+    if (!(kf >= 0.0f)) return 0;
+    if (kf > (float)kMaxLattice) return kMaxLattice;
+    return (int)kf;
+}
+
+// sin(h) of a horizon value: h is 0 or (2k + 1) * hori_acc up to the walk's
+// accumulated rounding, so the host table of glibc values is used (a 1-ulp
+// difference in h changes the SVF by ~1e-16).
+__device__ __forceinline__ double sin_horizon(const WalkParams& P, double h) {
+    if (!(h > 0.0)) return 0.0;
+    int k = (int)rint(h / (2.0 * P.hori_acc) - 0.5);
+    k = k < 0 ? 0 : (k > kMaxLattice ? kMaxLattice : k);
+    return P.sin_h[k];
+}
+
 __device__ __forceinline__ float sel3(float a, float b, float c, int k) {
     return k == 0 ? a : (k == 1 ? b : c);
 }
@@ -165,7 +191,7 @@ __device__ __forceinline__ bool leaf_hit(const Ray& r, const float4* __restrict_
 // then descend or pop.  Returns 0 while the query is open, 1 on a hit, 2
 // when the stack empties without one.  The stack keeps its first kLdsStack
 // entries in LDS at stk[k * BLOCK] and the rest in the private array ovf.
-template <bool COUNT, int BLOCK>
+template <bool COUNT, int BLOCK, bool MULTI>
 __device__ __forceinline__ int trav_step(const BvhNode* __restrict__ nodes,
                                          const float4* __restrict__ tris, int* stk, int* ovf,
                                          const Ray& r, int& node, int& sp, unsigned& nvisit,
@@ -194,23 +220,29 @@ __device__ __forceinline__ int trav_step(const BvhNode* __restrict__ nodes,
     bool h0 = tmin0 <= tmax0;
     bool h1 = tmin1 <= tmax1;
     const int r0 = __float_as_int(d.x), r1 = __float_as_int(d.y);
-    if (h0 && r0 < 0) {
-        if (r0 == ~(~r0 & kLeafFirstMask)) {  // single triangle, inline in slot 0
+    // leaf children: single triangles are inline (slot 0: d.zw e4 f4.xyz,
+    // slot 1: f4.w g4 h4); larger leaves (MULTI only) live in the tris array.
+    // One test site for both slots keeps the register footprint small.
+    const bool l0 = h0 && r0 < 0, l1 = h1 && r1 < 0;
+    if (l0 || l1) {
+#pragma unroll 1
+        for (int s = 0; s < 2; s++) {
+            const bool act = s == 0 ? l0 : l1;
+            if (!act) continue;
+            const int ref = s == 0 ? r0 : r1;
+            if (MULTI && ref != ~(~ref & kLeafFirstMask)) {
+                if (leaf_hit<COUNT>(r, tris, ref, ntest)) return 1;
+                continue;
+            }
             if (COUNT) ntest++;
-            if (tri_hit(r, d.z, d.w, e4.x, e4.y, e4.z, e4.w, f4.x, f4.y, f4.z)) return 1;
-        } else if (leaf_hit<COUNT>(r, tris, r0, ntest)) {
-            return 1;
+            const bool z = s == 0;
+            if (tri_hit(r, z ? d.z : f4.w, z ? d.w : g4.x, z ? e4.x : g4.y, z ? e4.y : g4.z,
+                        z ? e4.z : g4.w, z ? e4.w : h4.x, z ? f4.x : h4.y, z ? f4.y : h4.z,
+                        z ? f4.z : h4.w))
+                return 1;
         }
-        h0 = false;
-    }
-    if (h1 && r1 < 0) {
-        if (r1 == ~(~r1 & kLeafFirstMask)) {  // single triangle, inline in slot 1
-            if (COUNT) ntest++;
-            if (tri_hit(r, f4.w, g4.x, g4.y, g4.z, g4.w, h4.x, h4.y, h4.z, h4.w)) return 1;
-        } else if (leaf_hit<COUNT>(r, tris, r1, ntest)) {
-            return 1;
-        }
-        h1 = false;
+        h0 = h0 && r0 >= 0;
+        h1 = h1 && r1 >= 0;
     }
     if (h0 && h1) {
         bool first0 = tmin0 <= tmin1;
@@ -253,7 +285,8 @@ __device__ __forceinline__ bool occluded(const BvhNode* __restrict__ nodes,
             err |= 4;
             return true;
         }
-        int s = trav_step<COUNT, BLOCK>(nodes, tris, stk, ovf, r, node, sp, nvisit, ntest, err);
+        int s = trav_step<COUNT, BLOCK, true>(nodes, tris, stk, ovf, r, node, sp, nvisit, ntest,
+                                              err);
         if (s) return s == 1;
     }
 }
@@ -327,23 +360,26 @@ __global__ __launch_bounds__(BLOCK) void k_trace(const BvhNode* __restrict__ nod
                     h = e + a;
                 }
             } else {
-                // clip-aware climb from the horizontal direction
-                bool hit = cast(dir);
+                // clip-aware climb over the lattice from the horizontal direction
+                int k = 0;
+                bool hit = cast(lattice_dir(dir, n, P.elev_cs, 0));
                 if (!hit) {
-                    h = -a;
+                    h = 0.0;
                 } else {
-                    double ek = 0.0;
-                    D3 dk = dir;
-                    while (hit) {
-                        if (ek > kPi) {
+                    int top = 0;
+                    while (true) {
+                        if (k >= kMaxLattice) {  // straight up still hits: not a heightfield
                             err |= 2;
+                            top = k;
                             break;
                         }
-                        ek += 2.0 * a;
-                        dk = rodrigues(dk, ax, P.elev_sin_2ha, P.elev_cos_2ha);
-                        hit = cast(dk);
+                        k++;
+                        if (!cast(lattice_dir(dir, n, P.elev_cs, k))) {
+                            top = k - 1;
+                            break;
+                        }
                     }
-                    h = ek - a;
+                    h = (double)(2 * top + 1) * a;
                 }
             }
             if (h < 0.0) h = 0.0;  // (:737-741)
@@ -354,7 +390,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace(const BvhNode* __restrict__ nod
                 ksec = 0;
                 jsec++;
             }
-            double s = sin(h);  // SVF (:333-358)
+            double s = sin_horizon(P, h);  // SVF (:333-358)
             if (P.svf_type == 0)
                 svf += (1.0 - s);
             else if (P.svf_type == 1)
@@ -408,10 +444,10 @@ __global__ void k_clear_state(TraceState* ts) {
 // Per-lane FP64 search state kept in global memory between queries (it is
 // needed only when a query ends), so that the traversal loop runs with the
 // registers of the traversal alone.  Field f of lane g is spill[f * lanes + g].
-enum { S_DIR = 0, S_AX = 3, S_E = 6, S_SECTOR = 7, S_SVF = 8, S_DK = 9, S_EK = 12, S_FIELDS = 13 };
+enum { S_DIR = 0, S_AX = 3, S_E = 6, S_SECTOR = 7, S_SVF = 8, S_FIELDS = 9 };
 
-template <int MODE, bool COUNT, int BLOCK, int WAVES>
-__global__ __launch_bounds__(BLOCK, WAVES) void k_trace_pt(const BvhNode* __restrict__ nodes,
+template <int MODE, bool COUNT, int BLOCK, bool MULTI>
+__global__ __launch_bounds__(BLOCK, 4) void k_trace_pt(const BvhNode* __restrict__ nodes,
                                                     const float4* __restrict__ tris, int num_tri,
                                                     const CellFrame* __restrict__ cells,
                                                     const int* __restrict__ order, WalkParams P,
@@ -441,7 +477,7 @@ __global__ __launch_bounds__(BLOCK, WAVES) void k_trace_pt(const BvhNode* __rest
 
     // lane state kept in registers
     bool has_cell = false, live = false, last_hit = false;
-    int cell = 0, phase = P_AZSTART, j = 0, ksec = 0, jsec = 0;
+    int cell = 0, phase = P_AZSTART, j = 0, ksec = 0, jsec = 0, kk = 0;
     float o0 = 0.f, o1 = 0.f, o2 = 0.f;
     Ray ray = make_ray(0.f, 0.f, 0.f, 1.f, 0.f, 0.f, P.tfar);
     int node = kWideBit, sp = 0, steps = 0;
@@ -503,21 +539,25 @@ __global__ __launch_bounds__(BLOCK, WAVES) void k_trace_pt(const BvhNode* __rest
                 const double* fn = cells[cell].n;
                 const D3 n = {fn[0], fn[1], fn[2]};
                 D3 dir = {SV(S_DIR), SV(S_DIR + 1), SV(S_DIR + 2)};
-                D3 ax = {SV(S_AX), SV(S_AX + 1), SV(S_AX + 2)};
-                double e = SV(S_E), sector = SV(S_SECTOR), svf = SV(S_SVF);
-                D3 dk = {0.0, 0.0, 0.0};
-                double ek = 0.0;
-                if (MODE == 1) {
-                    dk = {SV(S_DK), SV(S_DK + 1), SV(S_DK + 2)};
-                    ek = SV(S_EK);
+                D3 ax = {0.0, 0.0, 0.0};
+                double e = 0.0;
+                if (MODE == 0) {
+                    ax = {SV(S_AX), SV(S_AX + 1), SV(S_AX + 2)};
+                    e = SV(S_E);
                 }
+                double sector = SV(S_SECTOR), svf = SV(S_SVF);
                 D3 q = dir;  // direction of the query to issue
                 bool issue = false;
                 while (true) {
                     if (phase == P_AZSTART) {
-                        ax = unit_cross(dir, n);
                         phase = P_FIRST;
-                        q = dir;
+                        if (MODE == 0) {
+                            ax = unit_cross(dir, n);
+                            q = dir;
+                        } else {
+                            kk = MODE == 1 ? 0 : start_index(P.pred[(size_t)cell * P.pred_stride + j]);
+                            q = lattice_dir(dir, n, P.elev_cs, kk);
+                        }
                         issue = true;
                         break;
                     }
@@ -563,31 +603,36 @@ __global__ __launch_bounds__(BLOCK, WAVES) void k_trace_pt(const BvhNode* __rest
                             }
                             h = e + a;
                         }
-                    } else {  // clip-aware climb from the horizontal direction
-                        if (phase == P_FIRST) {
-                            if (last_hit) {
+                    } else {
+                        // lattice search: climb while the query at index kk hits, descend
+                        // (learned start only) while it misses; the result is the top hit
+                        // index, or 0 deg when index 0 misses
+                        int top = -1;
+                        if (last_hit) {
+                            if (phase == P_DOWN) {
+                                top = kk;
+                            } else if (kk >= kMaxLattice) {
+                                err |= 2;  // straight up still hits: not a heightfield
+                                top = kk;
+                            } else {
                                 phase = P_UP;
-                                ek = 0.0;
-                                ek += 2.0 * a;
-                                dk = rodrigues(dir, ax, P.elev_sin_2ha, P.elev_cos_2ha);
-                                q = dk;
+                                kk++;
+                                q = lattice_dir(dir, n, P.elev_cs, kk);
                                 issue = true;
                                 break;
                             }
-                            h = -a;
                         } else {
-                            if (last_hit) {
-                                if (!(ek > kPi)) {
-                                    ek += 2.0 * a;
-                                    dk = rodrigues(dk, ax, P.elev_sin_2ha, P.elev_cos_2ha);
-                                    q = dk;
-                                    issue = true;
-                                    break;
-                                }
-                                err |= 2;
+                            if (phase == P_UP) {
+                                top = kk - 1;
+                            } else if (kk > 0) {
+                                phase = P_DOWN;
+                                kk--;
+                                q = lattice_dir(dir, n, P.elev_cs, kk);
+                                issue = true;
+                                break;
                             }
-                            h = ek - a;
                         }
+                        h = top >= 0 ? (double)(2 * top + 1) * a : 0.0;
                     }
                     // azimuth finished: clip, sector mean, SVF (:736-754)
                     if (h < 0.0) h = 0.0;
@@ -599,7 +644,7 @@ __global__ __launch_bounds__(BLOCK, WAVES) void k_trace_pt(const BvhNode* __rest
                         ksec = 0;
                         jsec++;
                     }
-                    const double sn = sin(h);
+                    const double sn = sin_horizon(P, h);
                     if (P.svf_type == 0)
                         svf += (1.0 - sn);
                     else if (P.svf_type == 1)
@@ -624,17 +669,13 @@ __global__ __launch_bounds__(BLOCK, WAVES) void k_trace_pt(const BvhNode* __rest
                     SV(S_DIR) = dir.x;
                     SV(S_DIR + 1) = dir.y;
                     SV(S_DIR + 2) = dir.z;
-                    SV(S_AX) = ax.x;
-                    SV(S_AX + 1) = ax.y;
-                    SV(S_AX + 2) = ax.z;
-                    SV(S_E) = e;
                     SV(S_SECTOR) = sector;
                     SV(S_SVF) = svf;
-                    if (MODE == 1) {
-                        SV(S_DK) = dk.x;
-                        SV(S_DK + 1) = dk.y;
-                        SV(S_DK + 2) = dk.z;
-                        SV(S_EK) = ek;
+                    if (MODE == 0) {
+                        SV(S_AX) = ax.x;
+                        SV(S_AX + 1) = ax.y;
+                        SV(S_AX + 2) = ax.z;
+                        SV(S_E) = e;
                     }
                 }
             }
@@ -649,8 +690,8 @@ __global__ __launch_bounds__(BLOCK, WAVES) void k_trace_pt(const BvhNode* __rest
             if (live) {
                 int s = 2;
                 if (have) {
-                    s = trav_step<COUNT, BLOCK>(nodes, tris, stk, ovf, ray, node, sp, nvisit, ntest,
-                                                err);
+                    s = trav_step<COUNT, BLOCK, MULTI>(nodes, tris, stk, ovf, ray, node, sp, nvisit,
+                                                       ntest, err);
                     // a BVH of n triangles has n - 1 interior nodes; more steps than
                     // that in one query can only come from a malformed tree
                     if (s == 0 && ++steps > (1 << 28)) {
@@ -681,7 +722,7 @@ __global__ __launch_bounds__(BLOCK, WAVES) void k_trace_pt(const BvhNode* __rest
     }
 }
 
-template <int MODE, bool COUNT, int BLOCK, int WAVES>
+template <int MODE, bool COUNT, int BLOCK, bool MULTI>
 int pt_grid(int ncell) {
     static const int occ_grid = []() {
         int dev = 0, cus = 0, per = 0;
@@ -689,7 +730,7 @@ int pt_grid(int ncell) {
         if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
             return 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                &per, reinterpret_cast<const void*>(&k_trace_pt<MODE, COUNT, BLOCK, WAVES>), BLOCK,
+                &per, reinterpret_cast<const void*>(&k_trace_pt<MODE, COUNT, BLOCK, MULTI>), BLOCK,
                 0) !=
             hipSuccess)
             return 0;
@@ -700,33 +741,31 @@ int pt_grid(int ncell) {
     return std::max(8, (g + 7) / 8 * 8);
 }
 
-template <int MODE, bool COUNT, int BLOCK, int WAVES>
+template <int MODE, bool COUNT, int BLOCK, bool MULTI>
 void launch_pt(const DeviceBvh& bvh, const CellFrame* cells, const int* order,
                const WalkParams& p, int refill, double* horizon, double* skyview,
                TraceState* ts, double* spill, hipStream_t st) {
-    hipLaunchKernelGGL((k_trace_pt<MODE, COUNT, BLOCK, WAVES>),
-                       dim3(pt_grid<MODE, COUNT, BLOCK, WAVES>(p.ncell)), dim3(BLOCK), 0, st,
+    hipLaunchKernelGGL((k_trace_pt<MODE, COUNT, BLOCK, MULTI>),
+                       dim3(pt_grid<MODE, COUNT, BLOCK, MULTI>(p.ncell)), dim3(BLOCK), 0, st,
                        bvh.nodes, bvh.tris, bvh.num_tri, cells, order, p, refill, horizon,
                        skyview, ts, spill);
 }
 
 template <int MODE, bool COUNT, int BLOCK>
 void launch_one(const DeviceBvh& bvh, const CellFrame* cells, const int* order,
-                const WalkParams& p, int kernel, int refill, double* horizon, double* skyview,
-                TraceState* ts, double* spill, hipStream_t st) {
+                const WalkParams& p, int kernel, bool multi, int refill, double* horizon,
+                double* skyview, TraceState* ts, double* spill, hipStream_t st) {
     if (kernel == 0) {
-        launch_pt<MODE, COUNT, BLOCK, 1>(bvh, cells, order, p, refill, horizon, skyview, ts,
-                                         spill, st);
+        if (multi)
+            launch_pt<MODE, COUNT, BLOCK, true>(bvh, cells, order, p, refill, horizon, skyview, ts,
+                                                spill, st);
+        else
+            launch_pt<MODE, COUNT, BLOCK, false>(bvh, cells, order, p, refill, horizon, skyview,
+                                                 ts, spill, st);
         return;
     }
-    if (kernel == 2 || kernel == 3) {  // persistent, occupancy target 4 / 5 waves per SIMD
-        if (kernel == 2)
-            launch_pt<MODE, COUNT, BLOCK, 4>(bvh, cells, order, p, refill, horizon, skyview, ts,
-                                             spill, st);
-        else
-            launch_pt<MODE, COUNT, BLOCK, 5>(bvh, cells, order, p, refill, horizon, skyview, ts,
-                                             spill, st);
-        return;
+    if constexpr (MODE == 2) {
+        return;  // the learned start exists only in the persistent kernel
     }
     TraceCounters* cnt = &ts->cnt;
     int nb = (p.ncell + BLOCK - 1) / BLOCK;
@@ -737,17 +776,17 @@ void launch_one(const DeviceBvh& bvh, const CellFrame* cells, const int* order,
 
 template <int MODE, bool COUNT>
 void launch_block(int block, const DeviceBvh& bvh, const CellFrame* cells, const int* order,
-                  const WalkParams& p, int kernel, int refill, double* horizon, double* skyview,
-                  TraceState* ts, double* spill, hipStream_t st) {
+                  const WalkParams& p, int kernel, bool multi, int refill, double* horizon,
+                  double* skyview, TraceState* ts, double* spill, hipStream_t st) {
     if (block == 64)
-        launch_one<MODE, COUNT, 64>(bvh, cells, order, p, kernel, refill, horizon, skyview, ts,
-                                    spill, st);
+        launch_one<MODE, COUNT, 64>(bvh, cells, order, p, kernel, multi, refill, horizon, skyview,
+                                    ts, spill, st);
     else if (block == 256)
-        launch_one<MODE, COUNT, 256>(bvh, cells, order, p, kernel, refill, horizon, skyview, ts,
-                                     spill, st);
+        launch_one<MODE, COUNT, 256>(bvh, cells, order, p, kernel, multi, refill, horizon,
+                                     skyview, ts, spill, st);
     else
-        launch_one<MODE, COUNT, 128>(bvh, cells, order, p, kernel, refill, horizon, skyview, ts,
-                                     spill, st);
+        launch_one<MODE, COUNT, 128>(bvh, cells, order, p, kernel, multi, refill, horizon,
+                                     skyview, ts, spill, st);
 }
 
 }  // namespace
@@ -766,23 +805,29 @@ int launch_trace(const DeviceBvh& bvh, const CellFrame* cells, const int* order,
                  const WalkParams& p, int mode, int kernel, int block, int refill, bool counters,
                  double* horizon, double* skyview, TraceState* ts, double* spill, hipStream_t st,
                  std::string& err) {
+    const bool multi = bvh.leaf_size > 1;
     if (refill < 1 || refill > 64) refill = 40;
     hipLaunchKernelGGL(k_clear_state, dim3(1), dim3(256), 0, st, ts);
-    if (mode == 0) {
-        if (counters)
-            launch_block<0, true>(block, bvh, cells, order, p, kernel, refill, horizon, skyview, ts,
-                                  spill, st);
-        else
-            launch_block<0, false>(block, bvh, cells, order, p, kernel, refill, horizon, skyview, ts,
-                                   spill, st);
-    } else {
-        if (counters)
-            launch_block<1, true>(block, bvh, cells, order, p, kernel, refill, horizon, skyview, ts,
-                                  spill, st);
-        else
-            launch_block<1, false>(block, bvh, cells, order, p, kernel, refill, horizon, skyview, ts,
-                                   spill, st);
+    if (mode == 2 && (kernel == 1 || !p.pred)) {
+        err = "the learned search needs the persistent kernel and predictions";
+        return -1;
     }
+#define HZ_LAUNCH(M)                                                                          \
+    do {                                                                                      \
+        if (counters)                                                                         \
+            launch_block<M, true>(block, bvh, cells, order, p, kernel, multi, refill, horizon,   \
+                                  skyview, ts, spill, st);                                    \
+        else                                                                                  \
+            launch_block<M, false>(block, bvh, cells, order, p, kernel, multi, refill, horizon,  \
+                                   skyview, ts, spill, st);                                   \
+    } while (0)
+    if (mode == 0)
+        HZ_LAUNCH(0);
+    else if (mode == 1)
+        HZ_LAUNCH(1);
+    else
+        HZ_LAUNCH(2);
+#undef HZ_LAUNCH
     HZ_HIP(hipGetLastError(), err);
     return 0;
 }

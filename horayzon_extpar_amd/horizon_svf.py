@@ -172,6 +172,27 @@ class ResidentDomain:
         self.num_hori = int(num_hori)
         return st.as_dict()
 
+    def set_predictor(self, predictor):
+        """Use ``predictor`` (learned.Predictor, kept alive by the caller) in
+        search_mode=2 runs."""
+        self._predictor = predictor
+        check(_lib.lib().hz_domain_set_predictor(self._h, predictor.handle if predictor
+                                                 else None))
+
+    def descriptors(self):
+        """Per-cell descriptors, BF16 bit patterns [ncell, 192]."""
+        out = np.empty((self.ncell, 192), np.uint16)
+        check(_lib.lib().hz_domain_descriptors(self._h, out.ctypes.data_as(
+            ctypes.POINTER(ctypes.c_uint16))))
+        return out
+
+    def predictions(self):
+        """Predicted horizon [ncell, 24] (deg) of the last search_mode=2 run."""
+        out = np.empty((self.ncell, 24), np.float32)
+        check(_lib.lib().hz_domain_predictions(self._h, out.ctypes.data_as(
+            ctypes.POINTER(ctypes.c_float))))
+        return out
+
     def fetch(self):
         horizon = np.empty((self.num_hori, self.ncell), np.float64)
         skyview = np.empty(self.ncell, np.float64)

@@ -28,17 +28,19 @@ extern "C" {
 /* Search strategies of the elevation-angle search. */
 #define HZ_MODE_WALK 0    /* the reference walk: carried elevation, 0.5 deg steps, -85 deg floor */
 #define HZ_MODE_CLIMB 1   /* clip-aware per-azimuth climb from 0 deg (independent azimuths) */
+#define HZ_MODE_LEARNED 2 /* lattice search started at a predicted horizon (needs a predictor
+                             and num_hori * refine_factor == 24) */
 
 typedef struct hz_options {
     int search_mode;      /* HZ_MODE_*; default HZ_MODE_WALK */
-    int leaf_size;        /* BVH leaf size 1..8; 0 = library default */
+    int leaf_size;        /* BVH leaf size 1..8; 0 = library default (1: every leaf is one
+                             triangle stored inline in its parent node) */
     int block_size;       /* trace kernel workgroup size (64, 128, 256); 0 = default */
     int collect_counters; /* 1: count BVH node visits and triangle tests (slower) */
     int cell_order;       /* 1: trace cells in Morton order of their position (default), 0: input order */
     int verbose;          /* 1: print the reference's progress lines to stderr */
     int trace_kernel;     /* 0: persistent lanes with per-XCD work queues (default),
-                             1: one lane per cell with azimuths in lock step,
-                             2 / 3: persistent, compiled for 4 / 5 waves per SIMD */
+                             1: one lane per cell with azimuths in lock step */
     int refill;           /* persistent kernel: refill lanes once fewer than this many of
                              the 64 lanes of a wavefront hold a live query (1..64; 0 = default) */
     int reserved[8];
@@ -61,7 +63,8 @@ typedef struct hz_stats {
     double t_download_ms;        /* device -> host copies */
     double t_total_ms;           /* wall time of the call */
     double lon0, lat0;           /* ENU origin [rad] */
-    double reserved[8];
+    double t_predict_ms;         /* device raster + descriptors + predictor (learned mode) */
+    double reserved[7];
 } hz_stats;
 
 /*
@@ -136,6 +139,49 @@ int hz_scene_export(const double* clon, const double* clat, const double* hsurf,
                     long long* num_triangles, long long* num_mesh_vertices, float* vert32,
                     int* tri, double* frame_n, double* frame_north, float* org,
                     double* lon0_lat0);
+
+/*
+ * Learned elevation bracket (no counterpart in the reference, which always
+ * runs the full search, mo_lradtopo_horayzon.cpp:490-555).  A predictor is
+ * a 192 -> 256 -> 256 -> 24 GELU network evaluated in BF16 on the matrix
+ * cores; its input is the per-cell descriptor (24 azimuths x 8 radial rings
+ * of maximum terrain elevation angle, 0.5 .. 40 km), its output the horizon
+ * per azimuth [deg].  HZ_MODE_LEARNED starts each azimuth's lattice search at
+ * the predicted elevation: it climbs while the query hits and descends while
+ * it misses, so a wrong prediction costs queries, never accuracy.
+ *
+ * params layout (hz_predictor_num_params() floats): w1[256][192], b1[256],
+ * w2[256][256], b2[256], w3[24][256], b3[24].
+ */
+typedef struct hz_predictor hz_predictor;
+long long hz_predictor_num_params(void);
+int hz_predictor_create(const float* params, hz_predictor** out);
+void hz_predictor_destroy(hz_predictor* p);
+/* The predictor must outlive the runs that use it. */
+int hz_domain_set_predictor(hz_domain* dom, const hz_predictor* p);
+/* Descriptors of all cells, BF16 bit patterns, desc[cell * 192 + az * 8 + ring]. */
+int hz_domain_descriptors(hz_domain* dom, unsigned short* desc);
+/* Predictions of the last learned run, pred[cell * 24 + az] [deg]. */
+int hz_domain_predictions(hz_domain* dom, float* pred);
+
+/*
+ * Native trainer of the predictor (Adam, mean squared error, cosine step
+ * decay), run on the device with the same MFMA kernels.  X: BF16 bit
+ * patterns [n][192]; T: targets [n][24] in deg.
+ */
+typedef struct hz_trainer hz_trainer;
+int hz_trainer_create(unsigned long long seed, hz_trainer** out);
+void hz_trainer_destroy(hz_trainer* t);
+int hz_trainer_set_weights(hz_trainer* t, const float* params);
+int hz_trainer_get_weights(hz_trainer* t, float* params);
+int hz_trainer_set_data(hz_trainer* t, const unsigned short* X, const float* T, int n);
+/* epoch_loss (optional, epochs floats): loss of the last batch of each epoch */
+int hz_trainer_fit(hz_trainer* t, int epochs, int batch, float lr, float* epoch_loss);
+/* One forward + backward pass on (X, T) without an update: gradients of the
+ * loss in params layout, and the loss.  n must be a multiple of 8. */
+int hz_trainer_gradients(hz_trainer* t, const unsigned short* X, const float* T, int n,
+                         float* grads, float* loss);
+int hz_trainer_predict(hz_trainer* t, const unsigned short* X, int n, float* out);
 
 /* Number of cells / mesh triangles of a resident domain. */
 int hz_domain_num_cell(const hz_domain* dom);

@@ -475,6 +475,9 @@ struct Setup {
     int ncell = 0, L = 0, rf = 1, nhori = 0, svf_type = 2;
     double hori_acc, thresh, azim_sin, azim_cos, es2, ec2, sh_s, sh_c;
     float tfar;
+    // cos, sin of k * 0.5 deg, k = 0 .. 180 (lattice directions of modes 1, 2)
+    double lat_c[181], lat_s[181];
+    const float* pred = nullptr;  // learned start [cell][24] deg (mode 2)
 };
 
 const double kRadEarth = 6371229.0;
@@ -524,6 +527,7 @@ void setup_domain(Setup& S, const double* clon, const double* clat, const double
     if (rf != 1) shift = -(deg2rad(360.0) / (2.0 * nhori)) + (deg2rad(360.0) / (2.0 * (double)S.L));
     sin_cos(-shift, &S.sh_s, &S.sh_c);
     S.tfar = (float)dist_search;
+    for (int k = 0; k <= 180; k++) sin_cos((double)k * (2.0 * S.hori_acc), &S.lat_s[k], &S.lat_c[k]);
 }
 
 inline bool cast(const Setup& S, const Frame& f, const V3& d, bool brute, long long* ntests) {
@@ -576,26 +580,49 @@ long long walk_cell(const Setup& S, int i, int mode, bool brute, double* horizon
                 h = e + a;
             }
         } else {
-            // horizontal direction of this azimuth, then climb from 0
-            V3 d0 = dir;
+            // lattice search over the horizontal direction dir of this azimuth:
+            // direction of index k = cos(k * 0.5 deg) dir + sin(k * 0.5 deg) n.
+            // mode 1 starts at k = 0, mode 2 at the index nearest to the
+            // predicted horizon; climb while hits, descend while misses
+            auto ldir = [&](int k) {
+                return V3{S.lat_c[k] * dir.x + S.lat_s[k] * f.n.x,
+                          S.lat_c[k] * dir.y + S.lat_s[k] * f.n.y,
+                          S.lat_c[k] * dir.z + S.lat_s[k] * f.n.z};
+            };
             int k = 0;
-            bool hit = cast(S, f, d0, brute, ntests);
-            rays++;
-            if (!hit) {
-                h = -a;
-            } else {
-                double ek = 0.0;
-                V3 dk = d0;
-                while (hit) {
-                    if (ek > kPi) break;
-                    k++;
-                    ek += 2.0 * a;
-                    dk = rodrigues(dk, ax, S.es2, S.ec2);
-                    hit = cast(S, f, dk, brute, ntests);
-                    rays++;
-                }
-                h = ek - a;
+            if (mode == 2) {
+                float kf = std::floor(S.pred[(size_t)i * 24 + j] * 2.0f + 0.5f);
+                k = !(kf >= 0.0f) ? 0 : (kf > 180.0f ? 180 : (int)kf);
             }
+            int top = -1;
+            bool hit = cast(S, f, ldir(k), brute, ntests);
+            rays++;
+            if (hit) {
+                while (true) {
+                    if (k >= 180) {  // guard for non-heightfield input
+                        top = k;
+                        break;
+                    }
+                    k++;
+                    hit = cast(S, f, ldir(k), brute, ntests);
+                    rays++;
+                    if (!hit) {
+                        top = k - 1;
+                        break;
+                    }
+                }
+            } else {
+                while (k > 0) {
+                    k--;
+                    hit = cast(S, f, ldir(k), brute, ntests);
+                    rays++;
+                    if (hit) {
+                        top = k;
+                        break;
+                    }
+                }
+            }
+            h = top >= 0 ? (double)(2 * top + 1) * a : 0.0;
         }
         if (h < 0.0) h = 0.0;
         hcell[j] = h;
@@ -707,7 +734,8 @@ int orc_enu(const double* clon, const double* clat, const double* hsurf, const d
 }
 
 // Full horizon / SVF computation with the reference signature plus:
-//   mode      0 = reference walk, 1 = clip-aware per-azimuth climb
+//   mode      0 = reference walk, 1 = clip-aware lattice climb from 0 deg,
+//             2 = lattice search started at pred[cell * 24 + azimuth] [deg]
 //   brute     1 = test every triangle for every ray (no BVH)
 //   nthreads  worker threads (<= 0: hardware concurrency)
 //   cell_begin/cell_end  restrict the work to a cell range (outputs for other
@@ -720,11 +748,13 @@ int orc_horizon_svf(const double* clon, const double* clat, const double* hsurf,
                     double* skyview, int ncell, int nvert, int nhori, int grid_type,
                     double dist_search, double ray_org_elev, int rf, int svf_type, int mode,
                     int brute, int nthreads, int cell_begin, int cell_end, double time_budget_s,
-                    orc_stats* st) {
+                    const float* pred, orc_stats* st) {
     double t0 = now_s();
     Setup S;
     setup_domain(S, clon, clat, hsurf, vlon, vlat, cov, ncell, nvert, nhori, grid_type,
                  dist_search, ray_org_elev, rf, svf_type, !brute);
+    S.pred = pred;
+    if (mode == 2 && (!pred || nhori * rf != 24)) return -1;
     double t1 = now_s();
     if (nthreads <= 0) nthreads = (int)std::max(1u, std::thread::hardware_concurrency());
     if (cell_end <= 0 || cell_end > ncell) cell_end = ncell;

@@ -100,9 +100,10 @@ def enu(dom, grid_type: int, ray_org_elev: float = 0.2):
 
 def horizon_svf(dom, num_hori=24, grid_type=1, dist_search=40000.0, ray_org_elev=0.2,
                 refine_factor=1, svf_type=2, mode=0, brute=False, nthreads=0,
-                cell_begin=0, cell_end=0, time_budget_s=0.0, out=None):
+                cell_begin=0, cell_end=0, time_budget_s=0.0, out=None, pred=None):
     """Oracle horizon [num_hori, ncell] (deg) and skyview [ncell].
-    mode 0 = reference walk, 1 = clip-aware per-azimuth climb."""
+    mode 0 = reference walk, 1 = clip-aware lattice climb, 2 = lattice search
+    started at pred [ncell, 24] (deg)."""
     clon, clat, hsurf, vlon, vlat, cov = _inputs(dom)
     ncell = clon.size
     if out is None:
@@ -111,12 +112,19 @@ def horizon_svf(dom, num_hori=24, grid_type=1, dist_search=40000.0, ray_org_elev
     else:
         horizon, skyview = out
     st = OracleStats()
-    lib().orc_horizon_svf(_p(clon, _D), _p(clat, _D), _p(hsurf, _D), _p(vlon, _D),
-                          _p(vlat, _D), _p(cov, _I), _p(horizon, _D), _p(skyview, _D),
-                          ncell, vlon.size, num_hori, grid_type, _D(dist_search),
-                          _D(ray_org_elev), refine_factor, svf_type, mode, int(brute),
-                          nthreads, cell_begin, cell_end, _D(time_budget_s),
-                          ctypes.byref(st))
+    pp = None
+    if pred is not None:
+        pred = np.ascontiguousarray(pred, np.float32)
+        assert pred.shape == (ncell, 24)
+        pp = _p(pred, _F)
+    rc = lib().orc_horizon_svf(_p(clon, _D), _p(clat, _D), _p(hsurf, _D), _p(vlon, _D),
+                               _p(vlat, _D), _p(cov, _I), _p(horizon, _D), _p(skyview, _D),
+                               ncell, vlon.size, num_hori, grid_type, _D(dist_search),
+                               _D(ray_org_elev), refine_factor, svf_type, mode, int(brute),
+                               nthreads, cell_begin, cell_end, _D(time_budget_s), pp,
+                               ctypes.byref(st))
+    if rc != 0:
+        raise ValueError("oracle: bad arguments")
     return horizon, skyview, st
 
 

@@ -1,0 +1,148 @@
+"""GPU tests of the learned elevation bracket.
+
+* The BF16 MFMA forward pass and the backward pass of the predictor match
+  the FP64 oracle (oracle/mlp_oracle.py, itself pinned by finite
+  differences) evaluated with the same BF16 roundings.
+* The native trainer lowers the loss.
+* search_mode=2 (lattice search started at the predicted horizon) equals the
+  CPU oracle's mode 2 bit for bit given the device's predictions; with a
+  constant prediction below 0 deg it is the plain climb, rays included; with
+  a wrong constant prediction it still returns the climb result (a miss
+  costs queries, never accuracy).
+* A trained predictor saves queries over the plain searches on a held-out
+  terrain while returning the same horizons away from the mesh border.
+"""
+
+import numpy as np
+import pytest
+
+import mlp_oracle as mo
+import oracle
+from horayzon_extpar_amd import learned, synthetic
+from horayzon_extpar_amd.horizon_svf import ResidentDomain
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
+
+
+def _batch(m=256, seed=0):
+    rng = np.random.default_rng(seed)
+    X = (rng.standard_normal((m, 192)) * 0.6).astype(np.float32)
+    T = (rng.standard_normal((m, 24)) * 3.0 + 2.0).astype(np.float32)
+    return learned.bf16_bits(X), T
+
+
+def test_forward_matches_oracle():
+    print("case mlp forward", flush=True)
+    tr = learned.Trainer(seed=3)
+    w = tr.get_weights()
+    Xb, _ = _batch()
+    Y = tr.predict(Xb)
+    ref = mo.forward(mo.unflatten(w), learned.bf16_to_f32(Xb), bf=True)["Y"]
+    err = np.abs(Y - ref).max()
+    print(f"forward max|dy|={err:.3g} (max|y|={np.abs(ref).max():.3g})", flush=True)
+    assert err <= 2e-2 * np.abs(ref).max() + 1e-3
+
+
+def test_gradients_match_oracle():
+    print("case mlp gradients", flush=True)
+    tr = learned.Trainer(seed=4)
+    w = tr.get_weights()
+    Xb, T = _batch(m=512, seed=1)
+    g, loss = tr.gradients(Xb, T)
+    gr, lref = mo.gradients(mo.unflatten(w), learned.bf16_to_f32(Xb), T, bf=True)
+    gg = learned.unflatten(g)
+    assert abs(loss - lref) <= 1e-3 * lref
+    for name, _ in mo.SHAPES:
+        r = _rel(gg[name], gr[name])
+        print(f"grad {name}: rel err {r:.2e}", flush=True)
+        assert r < 3e-2, name
+
+
+def test_training_reduces_loss():
+    print("case mlp training", flush=True)
+    rng = np.random.default_rng(5)
+    X = (rng.standard_normal((8192, 192)) * 0.5).astype(np.float32)
+    A = rng.standard_normal((192, 24)) / 8.0
+    T = (np.tanh(X @ A) * 5.0 + 3.0).astype(np.float32)
+    tr = learned.Trainer(seed=6)
+    tr.set_data(learned.bf16_bits(X), T)
+    loss = tr.fit(epochs=20, batch=512, lr=3e-3)
+    print(f"loss {loss[0]:.3f} -> {loss[-1]:.3f}", flush=True)
+    assert loss[-1] < 0.2 * loss[0]
+
+
+def _const_predictor(value):
+    p = {n: np.zeros(s, np.float32) for n, s in learned.SHAPES}
+    p["b3"][:] = value
+    return learned.Predictor(learned.flatten(p))
+
+
+def _small(seed):
+    return synthetic.config2(seed=seed, width_km=120.0, height_km=90.0)
+
+
+def test_constant_predictions_reproduce_climb():
+    print("case learned constant predictors", flush=True)
+    dom = _small(21)
+    low, high = _const_predictor(-3.0), _const_predictor(12.0)
+    with ResidentDomain(*dom.args(), grid_type=1) as rd:
+        st1 = rd.run(24, 40000.0, 1, 2, search_mode=1)
+        h1, s1 = rd.fetch()
+        rd.set_predictor(low)
+        stl = rd.run(24, 40000.0, 1, 2, search_mode=2)
+        hl, sl = rd.fetch()
+        rd.set_predictor(high)
+        sth = rd.run(24, 40000.0, 1, 2, search_mode=2)
+        hh, sh = rd.fetch()
+        P = rd.predictions()
+    assert np.array_equal(hl, h1) and np.array_equal(sl, s1)
+    assert stl["num_rays"] == st1["num_rays"]
+    assert np.allclose(P, 12.0)
+    # a wrong start costs queries, not accuracy (away from the border,
+    # where hits are monotone in elevation)
+    inner = (np.abs(dom.cx) < 20e3) & (np.abs(dom.cy) < 10e3)
+    assert np.array_equal(hh[:, inner], h1[:, inner])
+    assert sth["num_rays"] > st1["num_rays"]
+    # and bit-exact against the oracle's mode 2 with the same start
+    ho, so, sto = oracle.horizon_svf(dom, grid_type=1, mode=2, pred=P)
+    assert np.array_equal(hh, ho) and np.max(np.abs(sh - so)) <= 1e-12
+    assert sth["num_rays"] == sto.num_rays
+
+
+def test_trained_predictor_saves_queries():
+    print("case learned trained", flush=True)
+    cfg = lambda seed: synthetic.config2(seed=seed, width_km=200.0, height_km=150.0)  # noqa
+    w, loss, n = learned.train_predictor(seeds=(31, 32, 33), config=cfg, epochs=6,
+                                         batch=1024, lr=2e-3, max_cells=250_000)
+    assert loss[-1] < loss[0]
+    pred = learned.Predictor(w)
+    dom = cfg(34)  # held-out terrain
+    with ResidentDomain(*dom.args(), grid_type=1) as rd:
+        rd.set_predictor(pred)
+        st2 = rd.run(24, 40000.0, 1, 2, search_mode=2)
+        h2, s2 = rd.fetch()
+        P = rd.predictions()
+        st1 = rd.run(24, 40000.0, 1, 2, search_mode=1)
+        h1, s1 = rd.fetch()
+        st0 = rd.run(24, 40000.0, 1, 2, search_mode=0)
+        h0, s0 = rd.fetch()
+    print(f"rays: walk {st0['num_rays']} climb {st1['num_rays']} learned {st2['num_rays']}; "
+          f"trace ms: walk {st0['t_trace_ms']:.2f} climb {st1['t_trace_ms']:.2f} "
+          f"learned {st2['t_trace_ms']:.2f} (+predict {st2['t_predict_ms']:.2f})", flush=True)
+    err = np.abs(P - h1.T)
+    print(f"prediction |err| mean {err.mean():.3f} deg, <0.25 deg: {np.mean(err < 0.25):.3f}",
+          flush=True)
+    assert st2["num_rays"] < 0.8 * st1["num_rays"]
+    assert st2["num_rays"] < st0["num_rays"]
+    ho, so, _ = oracle.horizon_svf(dom, grid_type=1, mode=2, pred=P)
+    assert np.array_equal(h2, ho)
+    inner = (np.abs(dom.cx) < 55e3) & (np.abs(dom.cy) < 30e3)
+    assert np.array_equal(h2[:, inner], h1[:, inner])
+    assert np.allclose(h2[:, inner], h0[:, inner], atol=1e-9, rtol=0)
+    diff = np.abs(h2 - h0) > 1e-9
+    print(f"learned vs walk: {int(diff.sum())} of {diff.size} angles differ "
+          f"(all within {int(np.any(diff, axis=0).sum())} border cells)", flush=True)

@@ -57,7 +57,7 @@ def test_c1_climb_mode(c1):
 
 
 @pytest.mark.parametrize("kernel,block,refill",
-                         [(1, 128, 0), (0, 64, 64), (0, 256, 8), (2, 128, 0), (3, 128, 0)])
+                         [(1, 128, 0), (1, 64, 0), (0, 64, 64), (0, 256, 8), (0, 128, 1)])
 def test_kernel_variants_bit_exact(c1, kernel, block, refill):
     print(f"case c1 kernel={kernel} block={block} refill={refill}", flush=True)
     h, s = horizon_svf_comp_py(*c1.args(), 24, 1, 40000.0, 0.2, 1, 2, trace_kernel=kernel,
