@@ -39,9 +39,14 @@ def main():
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--kernel", type=int, default=0, help="0 persistent, 1 one lane per cell")
     ap.add_argument("--refill", type=int, default=0)
+    ap.add_argument("--morton", type=int, default=0, help="LBVH codes: 0 3D, 1 2D")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--weights", default="")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="first run one device-to-device copy of MIB MiB (a plain copy "
+                         "kernel of known traffic, for scaling rocprofv3 FETCH_SIZE / "
+                         "WRITE_SIZE)")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--cpu-threads", type=int, default=0)
     args = ap.parse_args()
@@ -55,13 +60,15 @@ def main():
     from horayzon_extpar_amd import synthetic
     from horayzon_extpar_amd.horizon_svf import ResidentDomain
 
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy)
     mode = {"walk": 0, "climb": 1}[args.mode]
     dom_in = synthetic.CONFIGS[args.config]()
     nhori, dist, rf, svf = 24, 40000.0, 1, 2
     print(f"[bench] workload {args.config}: ncell={dom_in.ncell} nvert={dom_in.nvert}",
           file=sys.stderr, flush=True)
     opts = dict(search_mode=mode, leaf_size=args.leaf or None, block_size=args.block or None,
-                trace_kernel=args.kernel, refill=args.refill or None)
+                trace_kernel=args.kernel, refill=args.refill or None, bvh_morton=args.morton)
     with ResidentDomain(*dom_in.args(), grid_type=1, ray_org_elev=0.2) as dom:
         cst = dom.create_stats
         for _ in range(args.warmup):
@@ -172,6 +179,21 @@ def learned_leg(dom, args, h_walk, s_walk, dom_in, plain_step_s):
             "max_abs_dsvf": float(np.abs(s - s_walk).max()),
             "prediction_mae_deg": float(err.mean()),
             "weights": os.path.relpath(path, ROOT)}
+
+
+def calibrate_copy(mib):
+    """One device-to-device copy of mib MiB through torch (reads and writes
+    exactly mib MiB each)."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    print(f"[bench] calibration copy: {mib} MiB read + {mib} MiB written", file=sys.stderr,
+          flush=True)
+    del a, b
 
 
 def border_affected(dom_in, num_hori, dist_search, margin_m=3000.0):

@@ -32,7 +32,7 @@ class HzOptions(ctypes.Structure):
                 ("block_size", ctypes.c_int), ("collect_counters", ctypes.c_int),
                 ("cell_order", ctypes.c_int), ("verbose", ctypes.c_int),
                 ("trace_kernel", ctypes.c_int), ("refill", ctypes.c_int),
-                ("reserved", ctypes.c_int * 8)]
+                ("bvh_morton", ctypes.c_int), ("reserved", ctypes.c_int * 7)]
 
 
 class HzStats(ctypes.Structure):

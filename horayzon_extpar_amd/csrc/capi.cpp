@@ -410,7 +410,8 @@ int hz_domain_run(hz_domain* d, int num_hori, double dist_search, int refine_fac
     RCHK(hipEventRecord(d->ev[0], st));
     d->bvh.nodes = d->d_nodes;
     d->bvh.tris = d->d_tris;
-    rc = lbvh_build(d->work, d->d_vert, d->d_tri, (int)d->ntri, leaf, st, d->bvh,
+    if (o.bvh_morton < 0 || o.bvh_morton > 1) return fail(HZ_EINVAL, "bvh_morton must be 0 or 1");
+    rc = lbvh_build(d->work, d->d_vert, d->d_tri, (int)d->ntri, leaf, o.bvh_morton, st, d->bvh,
                     o.collect_counters != 0, err);
     if (rc) return fail(rc == -3 ? HZ_ENOMEM : HZ_EDEVICE, err);
     RCHK(hipEventRecord(d->ev[1], st));

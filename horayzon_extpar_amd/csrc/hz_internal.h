@@ -120,7 +120,8 @@ struct LbvhWork {
     int* child = nullptr;   // 2 per interior node
     int* range = nullptr;   // 2 per interior node (first, last leaf)
     int* parent = nullptr;  // leaves [0, n), interior [n, 2n-1)
-    int* flags = nullptr;   // refit arrival counters, per interior node
+    void* boxes = nullptr;  // range-union work: sorted leaf boxes, prefix, suffix, sparse table
+    float* part = nullptr;  // per-workgroup centroid bounds
     unsigned* bounds = nullptr;  // 6 order-preserving encoded floats + depth + spare
     void* sort_tmp = nullptr;
     size_t sort_tmp_bytes = 0;
@@ -131,8 +132,10 @@ void lbvh_free(LbvhWork& w);
 // Builds the BVH of ntri triangles (vertex indices into vert, 3 floats per
 // vertex).  out.nodes / out.tris must hold max(1, ntri - 1) nodes and 3 * ntri
 // float4.  Enqueued on stream; depth is read back only when want_depth.
+// morton_mode 0: 3D codes in a cube of the largest extent, 1: 2D x/y codes
 int lbvh_build(LbvhWork& w, const float* vert, const int* tri, int ntri, int leaf_size,
-               hipStream_t stream, DeviceBvh& out, bool want_depth, std::string& err);
+               int morton_mode, hipStream_t stream, DeviceBvh& out, bool want_depth,
+               std::string& err);
 
 // Morton order of the cell ray origins (2D, x/y of the ENU frame).
 int order_cells(LbvhWork& w, const CellFrame* cells, int ncell, int* order, hipStream_t stream,

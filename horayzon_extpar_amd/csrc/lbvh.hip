@@ -1,17 +1,24 @@
 // On-device LBVH builder (replaces the reference's Embree rtcCommitScene,
 // mo_lradtopo_horayzon.cpp:395-432).
 //
-//   1. k_tri_bounds   triangle boxes and the centroid bounds of the scene
+//   1. k_tri_bounds   triangle boxes and per-workgroup centroid bounds,
+//      k_reduce_bounds  the scene's centroid bounds (no atomics)
 //   2. k_morton       63-bit Morton codes of the centroids in a cube of the
 //                     largest extent (a heightfield then splits mostly in x/y)
 //   3. rocprim radix sort of (code, triangle) pairs
 //   4. k_karras       Karras 2012 hierarchy ("Maximizing parallelism in the
 //                     construction of BVHs, octrees and k-d trees", HPG 2012),
 //                     duplicate codes broken by leaf index
-//   5. k_refit        bottom-up boxes, one arrival counter per interior node
-//   6. k_pack         64-byte traversal nodes holding both children's
-//                     conservative boxes; subtrees of <= leaf_size triangles
-//                     collapse into one leaf; triangles written in leaf order
+//   5. interior boxes without a bottom-up pass: a Karras node covers the
+//      contiguous leaf range [first, last], so its box is the union of that
+//      range.  k_block_scan forms prefix and suffix unions inside 256-leaf
+//      blocks, k_sparse_level a sparse table over block unions, and
+//      k_node_boxes answers every node's range in O(1) (short in-block
+//      ranges are unioned directly).  No atomics and no fences.
+//   6. k_pack         128-byte traversal nodes holding both children's
+//                     conservative boxes and inline single-triangle leaves;
+//                     subtrees of <= leaf_size triangles collapse into one
+//                     leaf; larger leaves' triangles written in leaf order
 //
 // Every box is grown by 1 m + 1e-5 |coordinate| so that the FP32 slab test
 // can never reject a ray that the watertight triangle test accepts: the
@@ -44,11 +51,14 @@ __global__ void k_init_bounds(unsigned* b) {
     else if (t < 8) b[t] = 0u;            // depth, spare
 }
 
+// Triangle boxes (original order) and, per workgroup, the min / max of the
+// triangle centroids: part[block * 8 + 0..2] = min, [3..5] = max.
 __global__ __launch_bounds__(kBlock) void k_tri_bounds(const float* __restrict__ vert,
                                                       const int* __restrict__ tri, int n,
                                                       float4* __restrict__ lo,
                                                       float4* __restrict__ hi,
-                                                      unsigned* __restrict__ bounds) {
+                                                      float* __restrict__ part) {
+    __shared__ float red[6][kBlock / 64];
     int i = blockIdx.x * kBlock + threadIdx.x;
     float cmin[3] = {3.0e38f, 3.0e38f, 3.0e38f}, cmax[3] = {-3.0e38f, -3.0e38f, -3.0e38f};
     if (i < n) {
@@ -65,21 +75,45 @@ __global__ __launch_bounds__(kBlock) void k_tri_bounds(const float* __restrict__
         lo[i] = make_float4(l[0], l[1], l[2], 0.0f);
         hi[i] = make_float4(h[0], h[1], h[2], 0.0f);
     }
-    // wave reduction then one atomic per wave
     for (int k = 0; k < 3; k++) {
         for (int off = 32; off > 0; off >>= 1) {
             cmin[k] = fminf(cmin[k], __shfl_xor(cmin[k], off, 64));
             cmax[k] = fmaxf(cmax[k], __shfl_xor(cmax[k], off, 64));
         }
     }
-    if ((threadIdx.x & 63) == 0) {
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
         for (int k = 0; k < 3; k++) {
-            if (cmin[k] <= cmax[k]) {
-                atomicMin(&bounds[k], f2ord(cmin[k]));
-                atomicMax(&bounds[3 + k], f2ord(cmax[k]));
-            }
+            red[k][w] = cmin[k];
+            red[3 + k][w] = cmax[k];
         }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const int k = threadIdx.x;
+        float v = red[k][0];
+        for (int q = 1; q < kBlock / 64; q++) v = k < 3 ? fminf(v, red[k][q]) : fmaxf(v, red[k][q]);
+        part[blockIdx.x * 8 + k] = v;
     }
+}
+
+// One workgroup: reduce nb partials into the encoded bounds b[0..5].
+__global__ __launch_bounds__(kBlock) void k_reduce_bounds(const float* __restrict__ part, int nb,
+                                                         unsigned* __restrict__ b) {
+    __shared__ float red[6][kBlock];
+    float v[6] = {3.0e38f, 3.0e38f, 3.0e38f, -3.0e38f, -3.0e38f, -3.0e38f};
+    for (int q = threadIdx.x; q < nb; q += kBlock)
+        for (int k = 0; k < 6; k++)
+            v[k] = k < 3 ? fminf(v[k], part[q * 8 + k]) : fmaxf(v[k], part[q * 8 + k]);
+    for (int k = 0; k < 6; k++) red[k][threadIdx.x] = v[k];
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s)
+            for (int k = 0; k < 6; k++)
+                red[k][threadIdx.x] = k < 3 ? fminf(red[k][threadIdx.x], red[k][threadIdx.x + s])
+                                            : fmaxf(red[k][threadIdx.x], red[k][threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) b[threadIdx.x] = f2ord(red[threadIdx.x][0]);
 }
 
 __device__ __forceinline__ uint64_t spread21(uint64_t x) {
@@ -92,9 +126,13 @@ __device__ __forceinline__ uint64_t spread21(uint64_t x) {
     return x;
 }
 
+__device__ __forceinline__ uint64_t spread32(uint64_t x);
+
+// mode 0: 3D codes (21 bits per axis) in a cube of the largest extent;
+// mode 1: 2D codes (32 bits per axis) of x / y only
 __global__ __launch_bounds__(kBlock) void k_morton(const float4* __restrict__ lo,
                                                   const float4* __restrict__ hi, int n,
-                                                  const unsigned* __restrict__ bounds,
+                                                  const unsigned* __restrict__ bounds, int mode,
                                                   uint64_t* __restrict__ keys,
                                                   int* __restrict__ vals) {
     int i = blockIdx.x * kBlock + threadIdx.x;
@@ -108,6 +146,17 @@ __global__ __launch_bounds__(kBlock) void k_morton(const float4* __restrict__ lo
     float4 l = lo[i], h = hi[i];
     float c[3] = {0.5f * (l.x + h.x), 0.5f * (l.y + h.y), 0.5f * (l.z + h.z)};
     uint64_t code = 0;
+    if (mode == 1) {
+        float e2 = fmaxf(ord2f(bounds[3]) - bmin[0], ord2f(bounds[4]) - bmin[1]);
+        float i2 = e2 > 0.0f ? 1.0f / e2 : 0.0f;
+        float qx = fminf(fmaxf((c[0] - bmin[0]) * i2, 0.0f), 1.0f);
+        float qy = fminf(fmaxf((c[1] - bmin[1]) * i2, 0.0f), 1.0f);
+        uint64_t ux = (uint64_t)fminf(qx * 4294967040.0f, 4294967040.0f);
+        uint64_t uy = (uint64_t)fminf(qy * 4294967040.0f, 4294967040.0f);
+        keys[i] = ((spread32(uy) << 1) | spread32(ux)) >> 1;  // 63 bits
+        vals[i] = i;
+        return;
+    }
     for (int k = 0; k < 3; k++) {
         float q = (c[k] - bmin[k]) * inv;
         q = fminf(fmaxf(q, 0.0f), 1.0f);
@@ -164,53 +213,86 @@ __global__ __launch_bounds__(kBlock) void k_karras(const uint64_t* __restrict__ 
     if (i == 0) parent[n] = -1;
 }
 
-__global__ __launch_bounds__(kBlock) void k_zero(int* __restrict__ p, int n) {
-    int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i < n) p[i] = 0;
+struct Box {
+    float4 lo, hi;
+};
+__device__ __forceinline__ Box box_empty() {
+    return {make_float4(3.0e38f, 3.0e38f, 3.0e38f, 0.f), make_float4(-3.0e38f, -3.0e38f, -3.0e38f, 0.f)};
+}
+__device__ __forceinline__ Box box_union(const Box& a, const Box& b) {
+    return {make_float4(fminf(a.lo.x, b.lo.x), fminf(a.lo.y, b.lo.y), fminf(a.lo.z, b.lo.z), 0.f),
+            make_float4(fmaxf(a.hi.x, b.hi.x), fmaxf(a.hi.y, b.hi.y), fmaxf(a.hi.z, b.hi.z), 0.f)};
 }
 
-// Bottom-up boxes.  One thread per leaf climbs while it is the second
-// arrival at a node (arrival counter per node).  Leaf boxes come from the
-// previous kernel.  The box a thread writes is read by the thread that
-// completes the parent: when the parent's leaf range lies inside this
-// workgroup's leaf chunk both threads belong to this workgroup and a
-// workgroup-scope fence orders the write and the read; only nodes whose
-// range crosses a chunk border (about two per chunk and level) pay for an
-// agent-scope fence, which on gfx950 writes back / invalidates the XCD's L2.
-__global__ __launch_bounds__(kBlock) void k_refit(const int* __restrict__ sorted, int n,
-                                                 float4* __restrict__ lo, float4* __restrict__ hi,
-                                                 const int* __restrict__ child,
-                                                 const int* __restrict__ range,
-                                                 const int* __restrict__ parent,
-                                                 int* __restrict__ flags) {
-    int leaf = blockIdx.x * kBlock + threadIdx.x;
-    if (leaf >= n) return;
-    const int chunk_lo = blockIdx.x * kBlock, chunk_hi = chunk_lo + kBlock - 1;
-    int p = parent[leaf];
-    bool wrote = false;
-    // a valid tree has < 128 levels (63 code bits + 32 index bits); the cap
-    // only bounds the loop
-    for (int guard = 0; p >= 0 && guard < 256; guard++) {
-        const bool local = range[2 * p] >= chunk_lo && range[2 * p + 1] <= chunk_hi;
-        if (wrote) {  // release the box written in the previous iteration
-            if (local) __threadfence_block(); else __threadfence();
-        }
-        if (atomicAdd(&flags[p], 1) == 0) return;  // first arrival: sibling not ready
-        if (local) __threadfence_block(); else __threadfence();  // acquire
-        float4 l[2], h[2];
-        for (int s = 0; s < 2; s++) {
-            int c = child[2 * p + s];
-            int slot = c < 0 ? sorted[~c] : n + c;
-            l[s] = lo[slot];
-            h[s] = hi[slot];
-        }
-        lo[n + p] = make_float4(fminf(l[0].x, l[1].x), fminf(l[0].y, l[1].y),
-                                fminf(l[0].z, l[1].z), 0.0f);
-        hi[n + p] = make_float4(fmaxf(h[0].x, h[1].x), fmaxf(h[0].y, h[1].y),
-                                fmaxf(h[0].z, h[1].z), 0.0f);
-        wrote = true;
-        p = parent[n + p];
+// Per 256-leaf block of the sorted leaf order: the leaf boxes (sorted), their
+// inclusive prefix and suffix unions within the block, and the block union
+// (level 0 of the sparse table).
+__global__ __launch_bounds__(kBlock) void k_block_scan(const int* __restrict__ sorted, int n,
+                                                      const float4* __restrict__ lo,
+                                                      const float4* __restrict__ hi,
+                                                      Box* __restrict__ leaf, Box* __restrict__ pre,
+                                                      Box* __restrict__ suf, Box* __restrict__ st0) {
+    __shared__ Box sp[kBlock], ss[kBlock];
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    Box bx = box_empty();
+    if (k < n) {
+        int t = sorted[k];
+        bx = {lo[t], hi[t]};
+        leaf[k] = bx;
     }
+    sp[threadIdx.x] = bx;
+    ss[threadIdx.x] = bx;
+    __syncthreads();
+    for (int d = 1; d < kBlock; d <<= 1) {
+        Box a = sp[threadIdx.x], b = ss[threadIdx.x];
+        if (threadIdx.x >= d) a = box_union(a, sp[threadIdx.x - d]);
+        if (threadIdx.x + d < kBlock) b = box_union(b, ss[threadIdx.x + d]);
+        __syncthreads();
+        sp[threadIdx.x] = a;
+        ss[threadIdx.x] = b;
+        __syncthreads();
+    }
+    if (k < n) {
+        pre[k] = sp[threadIdx.x];
+        suf[k] = ss[threadIdx.x];
+    }
+    if (threadIdx.x == 0) st0[blockIdx.x] = ss[0];
+}
+
+// dst[b] = src[b] U src[b + half] for b + half < nb (else src[b]).
+__global__ __launch_bounds__(kBlock) void k_sparse_level(const Box* __restrict__ src, int nb,
+                                                        int half, Box* __restrict__ dst) {
+    int b = blockIdx.x * kBlock + threadIdx.x;
+    if (b >= nb) return;
+    dst[b] = b + half < nb ? box_union(src[b], src[b + half]) : src[b];
+}
+
+// Box of every interior node from its leaf range; written to lo/hi[n + i].
+__global__ __launch_bounds__(kBlock) void k_node_boxes(const int* __restrict__ range, int n,
+                                                      const Box* __restrict__ leaf,
+                                                      const Box* __restrict__ pre,
+                                                      const Box* __restrict__ suf,
+                                                      const Box* __restrict__ st, int nb,
+                                                      float4* __restrict__ lo,
+                                                      float4* __restrict__ hi) {
+    int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n - 1) return;
+    const int a = range[2 * i], b = range[2 * i + 1];
+    const int ba = a / kBlock, bb = b / kBlock;
+    Box r;
+    if (ba == bb) {
+        r = leaf[a];
+        for (int k = a + 1; k <= b; k++) r = box_union(r, leaf[k]);
+    } else {
+        r = box_union(suf[a], pre[b]);
+        if (bb - ba > 1) {
+            const int l = ba + 1, h = bb - 1;
+            const int j = 31 - __clz(h - l + 1);  // floor(log2(length))
+            r = box_union(r, box_union(st[(size_t)j * nb + l], st[(size_t)j * nb + h - (1 << j) + 1]));
+        }
+    }
+    lo[n + i] = r.lo;
+    hi[n + i] = r.hi;
 }
 
 __device__ __forceinline__ float grow_lo(float v) { return v - (1.0f + 1e-5f * fabsf(v)); }
@@ -395,7 +477,11 @@ int lbvh_reserve(LbvhWork& w, int n, std::string& err) {
     HZ_HIP(hipMalloc(&w.child, nn * 2 * 4), err);
     HZ_HIP(hipMalloc(&w.range, nn * 2 * 4), err);
     HZ_HIP(hipMalloc(&w.parent, nn * 2 * 4), err);
-    HZ_HIP(hipMalloc(&w.flags, nn * 4), err);
+    const size_t nb = (nn + kBlock - 1) / kBlock;
+    int levels = 1;
+    while ((size_t)1 << levels < nb) levels++;
+    HZ_HIP(hipMalloc(&w.boxes, (3 * nn + (size_t)(levels + 1) * nb) * 2 * sizeof(float4)), err);
+    HZ_HIP(hipMalloc(&w.part, (nb + 1) * 8 * sizeof(float)), err);
     HZ_HIP(hipMalloc(&w.bounds, 16 * 4), err);
     size_t tmp = 0;
     HZ_HIP(rocprim::radix_sort_pairs(nullptr, tmp, w.keys[0], w.keys[1], w.vals[0], w.vals[1],
@@ -409,14 +495,14 @@ int lbvh_reserve(LbvhWork& w, int n, std::string& err) {
 
 void lbvh_free(LbvhWork& w) {
     void* p[] = {w.keys[0], w.keys[1], w.vals[0], w.vals[1], w.aabb_lo, w.aabb_hi, w.child,
-                 w.range, w.parent, w.flags, w.bounds, w.sort_tmp};
+                 w.range, w.parent, w.boxes, w.part, w.bounds, w.sort_tmp};
     for (void* q : p)
         if (q) (void)hipFree(q);
     w = LbvhWork();
 }
 
 int lbvh_build(LbvhWork& w, const float* vert, const int* tri, int n, int leaf_size,
-               hipStream_t st, DeviceBvh& out, bool want_depth, std::string& err) {
+               int morton_mode, hipStream_t st, DeviceBvh& out, bool want_depth, std::string& err) {
     out.num_tri = n;
     out.num_nodes = 0;
     out.depth = 0;
@@ -427,12 +513,13 @@ int lbvh_build(LbvhWork& w, const float* vert, const int* tri, int n, int leaf_s
     if (leaf_size > kMaxLeafSize) leaf_size = kMaxLeafSize;
     hipLaunchKernelGGL(k_init_bounds, dim3(1), dim3(64), 0, st, w.bounds);
     hipLaunchKernelGGL(k_tri_bounds, dim3(grid(n)), dim3(kBlock), 0, st, vert, tri, n, w.aabb_lo,
-                       w.aabb_hi, w.bounds);
+                       w.aabb_hi, w.part);
+    hipLaunchKernelGGL(k_reduce_bounds, dim3(1), dim3(kBlock), 0, st, w.part, grid(n), w.bounds);
     if (n == 1) {
         hipLaunchKernelGGL(k_pack_single, dim3(1), dim3(64), 0, st, w.aabb_lo, w.aabb_hi, vert,
                            tri, out.nodes);
         hipLaunchKernelGGL(k_morton, dim3(1), dim3(kBlock), 0, st, w.aabb_lo, w.aabb_hi, n,
-                           w.bounds, w.keys[0], w.vals[0]);
+                           w.bounds, morton_mode, w.keys[0], w.vals[0]);
         hipLaunchKernelGGL(k_tris, dim3(1), dim3(kBlock), 0, st, w.vals[0], n, vert, tri,
                            out.tris);
         out.num_nodes = 1;
@@ -440,7 +527,7 @@ int lbvh_build(LbvhWork& w, const float* vert, const int* tri, int n, int leaf_s
         return 0;
     }
     hipLaunchKernelGGL(k_morton, dim3(grid(n)), dim3(kBlock), 0, st, w.aabb_lo, w.aabb_hi, n,
-                       w.bounds, w.keys[0], w.vals[0]);
+                       w.bounds, morton_mode, w.keys[0], w.vals[0]);
     size_t tmp = w.sort_tmp_bytes;
     HZ_HIP(rocprim::radix_sort_pairs(w.sort_tmp, tmp, w.keys[0], w.keys[1], w.vals[0], w.vals[1],
                                      (unsigned)n, 0, 63, st),
@@ -449,9 +536,19 @@ int lbvh_build(LbvhWork& w, const float* vert, const int* tri, int n, int leaf_s
     const int* sorted = w.vals[1];
     hipLaunchKernelGGL(k_karras, dim3(grid(n - 1)), dim3(kBlock), 0, st, keys, n, w.child,
                        w.range, w.parent);
-    hipLaunchKernelGGL(k_zero, dim3(grid(n - 1)), dim3(kBlock), 0, st, w.flags, n - 1);
-    hipLaunchKernelGGL(k_refit, dim3(grid(n)), dim3(kBlock), 0, st, sorted, n, w.aabb_lo,
-                       w.aabb_hi, w.child, w.range, w.parent, w.flags);
+    {
+        const int nb = grid(n);
+        Box* boxes = reinterpret_cast<Box*>(w.boxes);
+        Box *leaf = boxes, *pre = boxes + n, *suf = boxes + 2 * (size_t)n, *stab = boxes + 3 * (size_t)n;
+        hipLaunchKernelGGL(k_block_scan, dim3(nb), dim3(kBlock), 0, st, sorted, n, w.aabb_lo,
+                           w.aabb_hi, leaf, pre, suf, stab);
+        int j = 1;
+        for (; (1 << j) <= nb; j++)
+            hipLaunchKernelGGL(k_sparse_level, dim3(grid(nb)), dim3(kBlock), 0, st,
+                               stab + (size_t)(j - 1) * nb, nb, 1 << (j - 1), stab + (size_t)j * nb);
+        hipLaunchKernelGGL(k_node_boxes, dim3(grid(n - 1)), dim3(kBlock), 0, st, w.range, n, leaf,
+                           pre, suf, stab, nb, w.aabb_lo, w.aabb_hi);
+    }
     hipLaunchKernelGGL(k_pack, dim3(grid(n - 1)), dim3(kBlock), 0, st, sorted, n, w.aabb_lo,
                        w.aabb_hi, w.child, w.range, leaf_size, vert, tri, out.nodes);
     hipLaunchKernelGGL(k_tris, dim3(grid(n)), dim3(kBlock), 0, st, sorted, n, vert, tri,

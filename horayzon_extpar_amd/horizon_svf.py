@@ -100,7 +100,8 @@ def validate(clon, clat, hsurf, vlon, vlat, cells_of_vertex, num_hori, grid_type
 def horizon_svf_comp_py(clon, clat, hsurf, vlon, vlat, cells_of_vertex, num_hori,
                         grid_type, dist_search, ray_org_elev, refine_factor, svf_type, *,
                         search_mode=None, leaf_size=None, block_size=None,
-                        collect_counters=None, trace_kernel=None, refill=None, stats=None):
+                        collect_counters=None, trace_kernel=None, refill=None,
+                        bvh_morton=None, stats=None):
     """Compute the terrain horizon [deg] and sky view factor [-].
 
     Same contract as the reference binding (horizon_svf.pyx:15-124):
@@ -119,7 +120,7 @@ def horizon_svf_comp_py(clon, clat, hsurf, vlon, vlat, cells_of_vertex, num_hori
                                      (clon, clat, hsurf, vlon, vlat))
     opt = default_options(search_mode=search_mode, leaf_size=leaf_size,
                           block_size=block_size, collect_counters=collect_counters,
-                          trace_kernel=trace_kernel, refill=refill)
+                          trace_kernel=trace_kernel, refill=refill, bvh_morton=bvh_morton)
     st = HzStats()
     rc = _lib.lib().horizon_svf_comp_ex(
         _ptr(clon, ctypes.c_double), _ptr(clat, ctypes.c_double),

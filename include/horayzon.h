@@ -43,7 +43,8 @@ typedef struct hz_options {
                              1: one lane per cell with azimuths in lock step */
     int refill;           /* persistent kernel: refill lanes once fewer than this many of
                              the 64 lanes of a wavefront hold a live query (1..64; 0 = default) */
-    int reserved[8];
+    int bvh_morton;       /* LBVH codes: 0 = 3D in a cube of the largest extent, 1 = 2D x/y */
+    int reserved[7];
 } hz_options;
 
 typedef struct hz_stats {

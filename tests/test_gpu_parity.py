@@ -68,13 +68,13 @@ def test_kernel_variants_bit_exact(c1, kernel, block, refill):
     assert np.max(np.abs(s - so)) <= SVF_TOL
 
 
-@pytest.mark.parametrize("leaf", [1, 8])
-def test_leaf_size_independent(c1, leaf):
+@pytest.mark.parametrize("leaf,morton", [(2, 0), (8, 0), (1, 1), (4, 1)])
+def test_tree_independent(c1, leaf, morton):
     # the occlusion answer is the OR over all triangles: any tree gives the same bits
-    print(f"case c1 leaf={leaf}", flush=True)
+    print(f"case c1 leaf={leaf} morton={morton}", flush=True)
     h4, s4 = horizon_svf_comp_py(*c1.args(), 24, 1, 40000.0, 0.2, 1, 2)
     h, s = horizon_svf_comp_py(*c1.args(), 24, 1, 40000.0, 0.2, 1, 2, leaf_size=leaf,
-                               block_size=64)
+                               block_size=64, bvh_morton=morton)
     assert np.array_equal(h, h4) and np.array_equal(s, s4)
 
 
