@@ -79,6 +79,13 @@ SIGNATURES = [
               ctypes.POINTER(ctypes.c_float), c_int_p, c_double_p, c_double_p,
               ctypes.POINTER(ctypes.c_float), c_double_p]),
     ("hz_domain_num_cell", ctypes.c_int, [ctypes.c_void_p]),
+    ("hz_domain_slope_aspect", ctypes.c_int, [ctypes.c_void_p, c_double_p, c_double_p]),
+    ("hz_batch_create", ctypes.c_int,
+     [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),
+    ("hz_batch_run", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+      ctypes.POINTER(HzOptions), ctypes.c_int, ctypes.POINTER(HzStats)]),
+    ("hz_batch_destroy", None, [ctypes.c_void_p]),
     ("hz_predictor_num_params", ctypes.c_longlong, []),
     ("hz_predictor_create", ctypes.c_int,
      [ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_void_p)]),

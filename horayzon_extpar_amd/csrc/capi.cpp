@@ -107,6 +107,9 @@ struct hz_domain {
     TraceState* d_state = nullptr;
     double* d_spill = nullptr;
     double2* d_elev_cs = nullptr;
+    int* d_adj_off = nullptr;  // cell -> adjacent triangles (slope / aspect)
+    int* d_adj = nullptr;
+    double* d_slope = nullptr;
     float bounds_xy[4] = {0, 0, 0, 0};  // scene x/y extent (raster of the descriptor)
     // learned mode
     const hz_predictor* predictor = nullptr;
@@ -143,7 +146,8 @@ void hz_domain_destroy(hz_domain* d) {
     if (d->stream) (void)hipStreamSynchronize(d->stream);
     void* bufs[] = {d->d_vert,   d->d_tri,     d->d_cells,   d->d_order, d->d_nodes,
                     d->d_tris,   d->d_horizon, d->d_skyview, d->d_state,
-                    d->d_spill,  d->d_elev_cs, d->d_desc,    d->d_pred};
+                    d->d_spill,  d->d_elev_cs, d->d_desc,    d->d_pred,
+                    d->d_adj_off, d->d_adj,    d->d_slope};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     lbvh_free(d->work);
@@ -249,6 +253,13 @@ int hz_domain_create(const double* clon, const double* clat, const double* hsurf
     DCHK(hipMalloc(&d->d_skyview, (size_t)num_cell * sizeof(double)));
     DCHK(hipMalloc(&d->d_state, sizeof(TraceState)));
     DCHK(hipMalloc(&d->d_spill, trace_spill_doubles(num_cell) * sizeof(double)));
+    DCHK(hipMalloc(&d->d_adj_off, hs.adj_off.size() * sizeof(int)));
+    DCHK(hipMalloc(&d->d_adj, std::max<size_t>(hs.adj.size(), 1) * sizeof(int)));
+    DCHK(hipMemcpyAsync(d->d_adj_off, hs.adj_off.data(), hs.adj_off.size() * sizeof(int),
+                        hipMemcpyHostToDevice, d->stream));
+    if (!hs.adj.empty())
+        DCHK(hipMemcpyAsync(d->d_adj, hs.adj.data(), hs.adj.size() * sizeof(int),
+                            hipMemcpyHostToDevice, d->stream));
     DCHK(hipMalloc(&d->d_elev_cs, (kMaxLattice + 1) * (sizeof(double2) + sizeof(double))));
     {
         std::vector<double2> cs(kMaxLattice + 1);
@@ -351,6 +362,22 @@ int hz_domain_descriptors(hz_domain* d, unsigned short* desc) {
     d->predictor = keep;
     if (rc) return fail(rc, err);
     RCHK(hipMemcpyAsync(desc, d->d_desc, (size_t)d->ncell * kIn * sizeof(uint16_t),
+                        hipMemcpyDeviceToHost, d->stream));
+    RCHK(hipStreamSynchronize(d->stream));
+    return HZ_OK;
+}
+
+int hz_domain_slope_aspect(hz_domain* d, double* slope, double* aspect) {
+    g_err.clear();
+    if (!d || !slope || !aspect) return fail(HZ_EINVAL, "NULL argument");
+    RCHK(hipSetDevice(d->device));
+    if (!d->d_slope) RCHK(hipMalloc(&d->d_slope, (size_t)d->ncell * 2 * sizeof(double)));
+    slope_aspect(d->d_vert, d->d_tri, d->d_adj_off, d->d_adj, d->d_cells, d->ncell, d->d_slope,
+                 d->d_slope + d->ncell, d->stream);
+    RCHK(hipGetLastError());
+    RCHK(hipMemcpyAsync(slope, d->d_slope, (size_t)d->ncell * sizeof(double),
+                        hipMemcpyDeviceToHost, d->stream));
+    RCHK(hipMemcpyAsync(aspect, d->d_slope + d->ncell, (size_t)d->ncell * sizeof(double),
                         hipMemcpyDeviceToHost, d->stream));
     RCHK(hipStreamSynchronize(d->stream));
     return HZ_OK;
@@ -562,6 +589,239 @@ void horizon_svf_comp(double* clon, double* clat, double* hsurf, double* vlon, d
                                  num_cell, num_vertex, num_hori, grid_type, dist_search,
                                  ray_org_elev, refine_factor, svf_type, nullptr, nullptr);
     if (rc) std::fprintf(stderr, "horizon_svf_comp failed (%d): %s\n", rc, g_err.c_str());
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batched domains
+// ---------------------------------------------------------------------------
+
+struct hz_batch {
+    std::vector<hz_domain*> doms;
+    int device = 0;
+    hipStream_t st = nullptr;
+    BatchDomain* d_table = nullptr;
+    TraceState* d_state = nullptr;
+    double* d_spill = nullptr;
+    size_t spill_cap = 0;
+    int total = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    // configuration the graph was captured for
+    int g_mode = -1, g_block = 0, g_refill = 0, g_chunk = 0, g_multi = -1, g_nhori = 0, g_rf = 0,
+        g_svf = 0;
+    double g_dist = 0;
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+};
+
+extern "C" {
+
+void hz_batch_destroy(hz_batch* b) {
+    if (!b) return;
+    (void)hipSetDevice(b->device);
+    if (b->st) (void)hipStreamSynchronize(b->st);
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    void* p[] = {b->d_table, b->d_state, b->d_spill};
+    for (void* q : p)
+        if (q) (void)hipFree(q);
+    for (auto& e : b->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (b->st) (void)hipStreamDestroy(b->st);
+    delete b;
+}
+
+#define BCHK(call)                                                                     \
+    do {                                                                               \
+        hipError_t e_ = (call);                                                        \
+        if (e_ != hipSuccess) {                                                        \
+            hz_batch_destroy(b);                                                       \
+            return fail(e_ == hipErrorOutOfMemory ? HZ_ENOMEM : HZ_EDEVICE,           \
+                        std::string(#call) + ": " + hipGetErrorString(e_));            \
+        }                                                                              \
+    } while (0)
+
+int hz_batch_create(hz_domain* const* doms, int n, hz_batch** out) {
+    g_err.clear();
+    if (!out || !doms || n < 1) return fail(HZ_EINVAL, "need at least one domain");
+    *out = nullptr;
+    hz_batch* b = new hz_batch();
+    long long total = 0;
+    for (int i = 0; i < n; i++) {
+        if (!doms[i]) {
+            delete b;
+            return fail(HZ_EINVAL, "NULL domain in batch");
+        }
+        if (i && doms[i]->device != doms[0]->device) {
+            delete b;
+            return fail(HZ_EINVAL, "all domains of a batch must live on one device");
+        }
+        b->doms.push_back(doms[i]);
+        total += doms[i]->ncell;
+    }
+    if (total > (long long)0x7fffffff - 4096) {
+        delete b;
+        return fail(HZ_EINVAL, "batch too large");
+    }
+    b->total = (int)total;
+    b->device = doms[0]->device;
+    BCHK(hipSetDevice(b->device));
+    BCHK(hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking));
+    for (auto& e : b->ev) BCHK(hipEventCreate(&e));
+    BCHK(hipMalloc(&b->d_table, n * sizeof(BatchDomain)));
+    BCHK(hipMalloc(&b->d_state, sizeof(TraceState)));
+    *out = b;
+    return HZ_OK;
+}
+
+#undef BCHK
+
+// One pass over all domains of the batch: per-domain LBVH build and cell
+// ordering (stream-ordered), then ONE persistent trace launch over every
+// cell -- replayed from a captured hipGraph when use_graph (captured on the
+// first call of a configuration).  Outputs stay in each domain's buffers;
+// hz_domain_fetch reads them.
+int hz_batch_run(hz_batch* b, int num_hori, double dist_search, int refine_factor, int svf_type,
+                 const hz_options* opt, int use_graph, hz_stats* stats) {
+    g_err.clear();
+    if (!b) return fail(HZ_EINVAL, "batch is NULL");
+    int rc = check_run_args(num_hori, dist_search, refine_factor, svf_type);
+    if (rc) return rc;
+    hz_options o;
+    if (opt)
+        o = *opt;
+    else
+        hz_default_options(&o);
+    if (o.search_mode != HZ_MODE_WALK && o.search_mode != HZ_MODE_CLIMB)
+        return fail(HZ_EINVAL, "batches run the walk or the climb search");
+    int block = o.block_size ? o.block_size : 128;
+    if (block != 64 && block != 128 && block != 256)
+        return fail(HZ_EINVAL, "block_size must be 64, 128 or 256");
+    int leaf = o.leaf_size ? o.leaf_size : 1;
+    if (leaf < 1 || leaf > kMaxLeafSize) return fail(HZ_EINVAL, "leaf_size must be in [1, 8]");
+    int refill = o.refill ? o.refill : 40;
+    if (refill < 1 || refill > 64) return fail(HZ_EINVAL, "refill must be in [1, 64]");
+    int chunk = o.reserved[0] ? o.reserved[0] : 64;  // queue chunk (cells per fetch)
+    if (chunk < 1 || chunk > 4096) return fail(HZ_EINVAL, "chunk must be in [1, 4096]");
+    RCHK(hipSetDevice(b->device));
+    hipStream_t st = b->st;
+    std::string err;
+    double t0 = now_ms();
+    RCHK(hipEventRecord(b->ev[0], st));
+    std::vector<BatchDomain> table(b->doms.size());
+    int start = 0;
+    for (size_t i = 0; i < b->doms.size(); i++) {
+        hz_domain* d = b->doms[i];
+        if (num_hori > d->out_nhori) {
+            if (d->d_horizon) RCHK(hipFree(d->d_horizon));
+            d->d_horizon = nullptr;
+            RCHK(hipMalloc(&d->d_horizon, (size_t)num_hori * d->ncell * sizeof(double)));
+            d->out_nhori = num_hori;
+        }
+        d->bvh.nodes = d->d_nodes;
+        d->bvh.tris = d->d_tris;
+        rc = lbvh_build(d->work, d->d_vert, d->d_tri, (int)d->ntri, leaf, o.bvh_morton, st, d->bvh,
+                        false, err);
+        if (rc) return fail(rc == -3 ? HZ_ENOMEM : HZ_EDEVICE, err);
+        if (o.cell_order) {
+            rc = order_cells(d->work, d->d_cells, d->ncell, d->d_order, st, err);
+            if (rc) return fail(HZ_EDEVICE, err);
+        }
+        BatchDomain& e = table[i];
+        e.nodes = d->d_nodes;
+        e.tris = d->d_tris;
+        e.cells = d->d_cells;
+        e.order = o.cell_order ? d->d_order : nullptr;
+        e.horizon = d->d_horizon;
+        e.skyview = d->d_skyview;
+        e.num_tri = (int)d->ntri;
+        e.ncell = d->ncell;
+        e.start = start;
+        e.pad = 0;
+        start += d->ncell;
+    }
+    RCHK(hipMemcpyAsync(b->d_table, table.data(), table.size() * sizeof(BatchDomain),
+                        hipMemcpyHostToDevice, st));
+    const bool multi = leaf > 1;
+    const int mode = o.search_mode;
+    size_t need = (size_t)batch_grid(mode, block, multi, b->total) * S_FIELDS_HOST;
+    if (need > b->spill_cap) {
+        if (b->d_spill) RCHK(hipFree(b->d_spill));
+        b->d_spill = nullptr;
+        RCHK(hipMalloc(&b->d_spill, need * sizeof(double)));
+        b->spill_cap = need;
+        if (b->exec) {  // the captured graph points at the old spill area
+            (void)hipGraphExecDestroy(b->exec);
+            (void)hipGraphDestroy(b->graph);
+            b->exec = nullptr;
+            b->graph = nullptr;
+        }
+    }
+    WalkParams p = make_walk_params(num_hori, refine_factor, svf_type, dist_search, 0);
+    p.elev_cs = b->doms[0]->d_elev_cs;
+    p.sin_h = reinterpret_cast<const double*>(b->doms[0]->d_elev_cs + (kMaxLattice + 1));
+    RCHK(hipEventRecord(b->ev[1], st));
+    if (use_graph) {
+        const bool same = b->exec && b->g_mode == mode && b->g_block == block &&
+                          b->g_refill == refill && b->g_chunk == chunk && b->g_multi == (int)multi &&
+                          b->g_nhori == num_hori && b->g_rf == refine_factor &&
+                          b->g_svf == svf_type && b->g_dist == dist_search;
+        if (!same) {
+            if (b->exec) {
+                (void)hipGraphExecDestroy(b->exec);
+                (void)hipGraphDestroy(b->graph);
+                b->exec = nullptr;
+                b->graph = nullptr;
+            }
+            RCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            rc = launch_trace_batch(b->d_table, (int)b->doms.size(), b->total, multi, p, mode, block,
+                                    refill, chunk, b->d_state, b->d_spill, st, err);
+            hipGraph_t g = nullptr;
+            hipError_t ec = hipStreamEndCapture(st, &g);
+            if (rc || ec != hipSuccess) {
+                if (g) (void)hipGraphDestroy(g);
+                return fail(HZ_EDEVICE, rc ? err : std::string("hipStreamEndCapture failed"));
+            }
+            b->graph = g;
+            RCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+            b->g_mode = mode;
+            b->g_block = block;
+            b->g_refill = refill;
+            b->g_chunk = chunk;
+            b->g_multi = multi;
+            b->g_nhori = num_hori;
+            b->g_rf = refine_factor;
+            b->g_svf = svf_type;
+            b->g_dist = dist_search;
+        }
+        RCHK(hipGraphLaunch(b->exec, st));
+    } else {
+        rc = launch_trace_batch(b->d_table, (int)b->doms.size(), b->total, multi, p, mode, block,
+                                refill, chunk, b->d_state, b->d_spill, st, err);
+        if (rc) return fail(HZ_EDEVICE, err);
+    }
+    RCHK(hipEventRecord(b->ev[2], st));
+    TraceCounters c;
+    RCHK(hipMemcpyAsync(&c, &b->d_state->cnt, sizeof(c), hipMemcpyDeviceToHost, st));
+    RCHK(hipStreamSynchronize(st));
+    float ms_build = 0, ms_trace = 0;
+    RCHK(hipEventElapsedTime(&ms_build, b->ev[0], b->ev[1]));
+    RCHK(hipEventElapsedTime(&ms_trace, b->ev[1], b->ev[2]));
+    for (hz_domain* d : b->doms) {
+        d->last_nhori = num_hori;
+        d->ran = c.error == 0;
+    }
+    if (stats) {
+        std::memset(stats, 0, sizeof(*stats));
+        stats->num_rays = (long long)c.rays;
+        stats->kernel_error = (int)c.error;
+        stats->t_bvh_ms = ms_build;
+        stats->t_trace_ms = ms_trace;
+        stats->t_total_ms = now_ms() - t0;
+    }
+    if (c.error) return fail(HZ_EKERNEL, "trace kernel exceeded its traversal stack or walk limit");
+    return HZ_OK;
 }
 
 }  // extern "C"

@@ -96,6 +96,8 @@ struct HostScene {
     std::vector<float> vert32;  // 3 floats per mesh vertex (ENU, FP32)
     std::vector<int> tri;       // 3 vertex indices per triangle
     std::vector<CellFrame> frame;
+    std::vector<int> adj_off;   // CSR: triangles with cell c as a corner are
+    std::vector<int> adj;       //      adj[adj_off[c] .. adj_off[c + 1])
     long long num_mesh_vertices = 0;
     double lon0 = 0.0, lat0 = 0.0;
 };
@@ -106,6 +108,11 @@ int prologue(const double* clon, const double* clat, const double* hsurf, const 
 
 WalkParams make_walk_params(int num_hori, int refine_factor, int svf_type, double dist_search,
                             int ncell);
+
+// Slope / aspect [deg] per cell (terrain.hip)
+void slope_aspect(const float* vert, const int* tri, const int* off, const int* adj,
+                  const CellFrame* cells, int ncell, double* slope, double* aspect,
+                  hipStream_t st);
 
 // ---------------------------------------------------------------------------
 // LBVH builder (lbvh.hip)
@@ -166,6 +173,29 @@ struct TraceState {
 //         1 = one lane per cell, azimuths in lock step
 // refill: the persistent kernel leaves its traversal loop to refill lanes
 //         once fewer than `refill` of the 64 lanes hold a live ray
+// One domain of a batched launch (device table entry).
+struct BatchDomain {
+    const BvhNode* nodes;
+    const float4* tris;
+    const CellFrame* cells;
+    const int* order;
+    double* horizon;
+    double* skyview;
+    int num_tri;
+    int ncell;
+    int start;  // global work index of the domain's first cell
+    int pad;
+};
+
+// One persistent launch over the cells of ndom domains sharing the search
+// parameters p (p.ncell unused); total = sum of their cells.  mode 0 or 1.
+int launch_trace_batch(const BatchDomain* doms, int ndom, int total, bool multi,
+                       const WalkParams& p, int mode, int block, int refill, int chunk,
+                       TraceState* ts, double* spill, hipStream_t st, std::string& err);
+// lanes of the batched launch (also settles its cached occupancy query, which
+// must not run inside a stream capture)
+int batch_grid(int mode, int block, bool multi, int total);
+
 // spill: per-lane FP64 search state of the persistent kernel, at least
 // trace_spill_doubles(...) doubles
 int launch_trace(const DeviceBvh& bvh, const CellFrame* cells, const int* order,
@@ -173,6 +203,7 @@ int launch_trace(const DeviceBvh& bvh, const CellFrame* cells, const int* order,
                  bool counters, double* horizon, double* skyview, TraceState* state,
                  double* spill, hipStream_t stream, std::string& err);
 size_t trace_spill_doubles(int ncell);
+constexpr int S_FIELDS_HOST = 9;  // FP64 fields of the per-lane search spill
 
 }  // namespace hz
 

@@ -277,6 +277,23 @@ int prologue(const double* clon, const double* clat, const double* hsurf, const 
             f.pad = 0;
         }
     });
+    // --- cell -> adjacent triangles (slope / aspect) -------------------------
+    out.adj_off.assign((size_t)ncell + 1, 0);
+    for (long long t = 0; t < ntri; t++)
+        for (int k = 0; k < 3; k++) {
+            int v = out.tri[3 * t + k];
+            if (v < ncell) out.adj_off[v + 1]++;
+        }
+    for (int c = 0; c < ncell; c++) out.adj_off[c + 1] += out.adj_off[c];
+    out.adj.assign((size_t)out.adj_off[ncell], 0);
+    {
+        std::vector<int> fill(out.adj_off.begin(), out.adj_off.end() - 1);
+        for (long long t = 0; t < ntri; t++)
+            for (int k = 0; k < 3; k++) {
+                int v = out.tri[3 * t + k];
+                if (v < ncell) out.adj[fill[v]++] = (int)t;
+            }
+    }
     out.num_mesh_vertices = nmv;
     return 0;
 }

@@ -445,25 +445,31 @@ __global__ void k_clear_state(TraceState* ts) {
 // needed only when a query ends), so that the traversal loop runs with the
 // registers of the traversal alone.  Field f of lane g is spill[f * lanes + g].
 enum { S_DIR = 0, S_AX = 3, S_E = 6, S_SECTOR = 7, S_SVF = 8, S_FIELDS = 9 };
+static_assert(S_FIELDS == S_FIELDS_HOST, "spill layout");
 
-template <int MODE, bool COUNT, int BLOCK, bool MULTI>
+// BATCH: the cells of several domains (same search parameters) form one
+// index space; doms[d] covers global indices [doms[d].start, doms[d + 1].start)
+// and each lane follows its own cell's domain (scene, outputs).
+template <int MODE, bool COUNT, int BLOCK, bool MULTI, bool BATCH>
 __global__ __launch_bounds__(BLOCK, 4) void k_trace_pt(const BvhNode* __restrict__ nodes,
                                                     const float4* __restrict__ tris, int num_tri,
                                                     const CellFrame* __restrict__ cells,
                                                     const int* __restrict__ order, WalkParams P,
-                                                    int refill, double* __restrict__ horizon,
+                                                    int refill, int chunk,
+                                                    double* __restrict__ horizon,
                                                     double* __restrict__ skyview,
                                                     TraceState* __restrict__ ts,
-                                                    double* __restrict__ spill) {
+                                                    double* __restrict__ spill,
+                                                    const BatchDomain* __restrict__ doms, int ndom,
+                                                    int total) {
     __shared__ int stk_all[kLdsStack * BLOCK];
     int* stk = stk_all + threadIdx.x;
     int ovf[kStackMax - kLdsStack];
     const int lane = threadIdx.x & 63;
     const unsigned long long lt_mask = (1ull << lane) - 1ull;
-    const int N = P.ncell;
+    const int N = BATCH ? total : P.ncell;  // size of the work index space
     const int L = P.num_hori * P.refine_factor;
     const double a = P.hori_acc;
-    const bool have = num_tri > 0;
     int* queue = ts->queue;
     const size_t lanes = (size_t)gridDim.x * BLOCK;
     double* sv = spill + (size_t)blockIdx.x * BLOCK + threadIdx.x;
@@ -483,6 +489,15 @@ __global__ __launch_bounds__(BLOCK, 4) void k_trace_pt(const BvhNode* __restrict
     int node = kWideBit, sp = 0, steps = 0;
     unsigned rays = 0, nvisit = 0, ntest = 0;
     int err = 0;
+    // the lane's domain (batch) or the single domain
+    int dom = 0;
+    const BvhNode* lnodes = nodes;
+    const float4* ltris = tris;
+    bool have = num_tri > 0;
+    const CellFrame* lcells = cells;
+    double* lhorizon = horizon;
+    double* lskyview = skyview;
+    int lncell = P.ncell;
 
     while (true) {
         // ---------------- refill ----------------
@@ -495,11 +510,11 @@ __global__ __launch_bounds__(BLOCK, 4) void k_trace_pt(const BvhNode* __restrict
                         const int rlo = (int)(((long long)cur * N) >> 3);
                         const int rhi = (int)(((long long)(cur + 1) * N) >> 3);
                         int old = 0;
-                        if (lane == 0) old = atomicAdd(&queue[cur * kQueueStride], kChunk);
+                        if (lane == 0) old = atomicAdd(&queue[cur * kQueueStride], chunk);
                         old = __shfl(old, 0, 64);
                         if (rlo + old < rhi) {
                             qb = rlo + old;
-                            qe = min(qb + kChunk, rhi);
+                            qe = min(qb + chunk, rhi);
                             break;
                         }
                         cur = (cur + 1) & 7;
@@ -511,9 +526,29 @@ __global__ __launch_bounds__(BLOCK, 4) void k_trace_pt(const BvhNode* __restrict
                     const int avail = qe - qb;
                     const int rank = __popcll(need & lt_mask);
                     if (want && rank < avail) {
-                        cell = order ? order[qb + rank] : qb + rank;
+                        const int g = qb + rank;
+                        if (BATCH) {
+                            int lo = 0, hi = ndom - 1;  // last domain with start <= g
+                            while (lo < hi) {
+                                const int mid = (lo + hi + 1) >> 1;
+                                if (doms[mid].start <= g) lo = mid; else hi = mid - 1;
+                            }
+                            dom = lo;
+                            const BatchDomain& bd = doms[dom];
+                            const int l = g - bd.start;
+                            cell = bd.order ? bd.order[l] : l;
+                            lnodes = bd.nodes;
+                            ltris = bd.tris;
+                            have = bd.num_tri > 0;
+                            lcells = bd.cells;
+                            lhorizon = bd.horizon;
+                            lskyview = bd.skyview;
+                            lncell = bd.ncell;
+                        } else {
+                            cell = order ? order[g] : g;
+                        }
                         has_cell = true;
-                        const CellFrame f = cells[cell];
+                        const CellFrame f = lcells[cell];
                         const D3 n = {f.n[0], f.n[1], f.n[2]};
                         const D3 north = {f.north[0], f.north[1], f.north[2]};
                         o0 = f.org[0];
@@ -536,7 +571,7 @@ __global__ __launch_bounds__(BLOCK, 4) void k_trace_pt(const BvhNode* __restrict
             }
             if (has_cell && !live) {
                 // consume the last answer and issue the next query of this cell
-                const double* fn = cells[cell].n;
+                const double* fn = lcells[cell].n;
                 const D3 n = {fn[0], fn[1], fn[2]};
                 D3 dir = {SV(S_DIR), SV(S_DIR + 1), SV(S_DIR + 2)};
                 D3 ax = {0.0, 0.0, 0.0};
@@ -555,7 +590,8 @@ __global__ __launch_bounds__(BLOCK, 4) void k_trace_pt(const BvhNode* __restrict
                             ax = unit_cross(dir, n);
                             q = dir;
                         } else {
-                            kk = MODE == 1 ? 0 : start_index(P.pred[(size_t)cell * P.pred_stride + j]);
+                            kk = MODE == 1 ? 0
+                                           : start_index(P.pred[(size_t)cell * P.pred_stride + j]);
                             q = lattice_dir(dir, n, P.elev_cs, kk);
                         }
                         issue = true;
@@ -638,7 +674,7 @@ __global__ __launch_bounds__(BLOCK, 4) void k_trace_pt(const BvhNode* __restrict
                     if (h < 0.0) h = 0.0;
                     sector += h;
                     if (++ksec == P.refine_factor) {
-                        horizon[(size_t)jsec * N + cell] =
+                        lhorizon[(size_t)jsec * lncell + cell] =
                             ((sector / kPi) * 180.0) / (double)P.refine_factor;
                         sector = 0.0;
                         ksec = 0;
@@ -653,7 +689,7 @@ __global__ __launch_bounds__(BLOCK, 4) void k_trace_pt(const BvhNode* __restrict
                         svf += (1.0 - (sn * sn * sn));
                     dir = rodrigues(dir, n, -P.azim_sin, P.azim_cos);  // (:550-551)
                     if (++j == L) {
-                        skyview[cell] = svf / (double)L;
+                        lskyview[cell] = svf / (double)L;
                         has_cell = false;
                         break;
                     }
@@ -690,7 +726,7 @@ __global__ __launch_bounds__(BLOCK, 4) void k_trace_pt(const BvhNode* __restrict
             if (live) {
                 int s = 2;
                 if (have) {
-                    s = trav_step<COUNT, BLOCK, MULTI>(nodes, tris, stk, ovf, ray, node, sp, nvisit,
+                    s = trav_step<COUNT, BLOCK, MULTI>(lnodes, ltris, stk, ovf, ray, node, sp, nvisit,
                                                        ntest, err);
                     // a BVH of n triangles has n - 1 interior nodes; more steps than
                     // that in one query can only come from a malformed tree
@@ -722,7 +758,7 @@ __global__ __launch_bounds__(BLOCK, 4) void k_trace_pt(const BvhNode* __restrict
     }
 }
 
-template <int MODE, bool COUNT, int BLOCK, bool MULTI>
+template <int MODE, bool COUNT, int BLOCK, bool MULTI, bool BATCH>
 int pt_grid(int ncell) {
     static const int occ_grid = []() {
         int dev = 0, cus = 0, per = 0;
@@ -730,8 +766,8 @@ int pt_grid(int ncell) {
         if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
             return 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                &per, reinterpret_cast<const void*>(&k_trace_pt<MODE, COUNT, BLOCK, MULTI>), BLOCK,
-                0) !=
+                &per, reinterpret_cast<const void*>(&k_trace_pt<MODE, COUNT, BLOCK, MULTI, BATCH>),
+                BLOCK, 0) !=
             hipSuccess)
             return 0;
         return std::max(1, per) * std::max(1, cus);
@@ -745,10 +781,19 @@ template <int MODE, bool COUNT, int BLOCK, bool MULTI>
 void launch_pt(const DeviceBvh& bvh, const CellFrame* cells, const int* order,
                const WalkParams& p, int refill, double* horizon, double* skyview,
                TraceState* ts, double* spill, hipStream_t st) {
-    hipLaunchKernelGGL((k_trace_pt<MODE, COUNT, BLOCK, MULTI>),
-                       dim3(pt_grid<MODE, COUNT, BLOCK, MULTI>(p.ncell)), dim3(BLOCK), 0, st,
-                       bvh.nodes, bvh.tris, bvh.num_tri, cells, order, p, refill, horizon,
-                       skyview, ts, spill);
+    hipLaunchKernelGGL((k_trace_pt<MODE, COUNT, BLOCK, MULTI, false>),
+                       dim3(pt_grid<MODE, COUNT, BLOCK, MULTI, false>(p.ncell)), dim3(BLOCK), 0,
+                       st, bvh.nodes, bvh.tris, bvh.num_tri, cells, order, p, refill, kChunk,
+                       horizon, skyview, ts, spill, nullptr, 0, 0);
+}
+
+template <int MODE, int BLOCK, bool MULTI>
+void launch_pt_batch(const BatchDomain* doms, int ndom, int total, const WalkParams& p,
+                     int refill, int chunk, TraceState* ts, double* spill, hipStream_t st) {
+    hipLaunchKernelGGL((k_trace_pt<MODE, false, BLOCK, MULTI, true>),
+                       dim3(pt_grid<MODE, false, BLOCK, MULTI, true>(total)), dim3(BLOCK), 0, st,
+                       nullptr, nullptr, 0, nullptr, nullptr, p, refill, chunk, nullptr, nullptr,
+                       ts, spill, doms, ndom, total);
 }
 
 template <int MODE, bool COUNT, int BLOCK>
@@ -789,6 +834,21 @@ void launch_block(int block, const DeviceBvh& bvh, const CellFrame* cells, const
                                      skyview, ts, spill, st);
 }
 
+template <int MODE>
+void batch_block(int block, bool multi, const BatchDomain* doms, int ndom, int total,
+                 const WalkParams& p, int refill, int chunk, TraceState* ts, double* spill,
+                 hipStream_t st) {
+#define HZ_B(BL, MU) launch_pt_batch<MODE, BL, MU>(doms, ndom, total, p, refill, chunk, ts, spill, st)
+    if (block == 64) {
+        if (multi) HZ_B(64, true); else HZ_B(64, false);
+    } else if (block == 256) {
+        if (multi) HZ_B(256, true); else HZ_B(256, false);
+    } else {
+        if (multi) HZ_B(128, true); else HZ_B(128, false);
+    }
+#undef HZ_B
+}
+
 }  // namespace
 
 size_t trace_spill_doubles(int ncell) {
@@ -799,6 +859,39 @@ size_t trace_spill_doubles(int ncell) {
         (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     size_t lanes = std::min((size_t)cus * 2048, (size_t)ncell + 8 * 256);
     return lanes * S_FIELDS;
+}
+
+int batch_grid(int mode, int block, bool multi, int total) {
+#define HZ_G(M, BL, MU) return pt_grid<M, false, BL, MU, true>(total) * BL
+#define HZ_GB(M)                                                    \
+    if (block == 64) {                                              \
+        if (multi) HZ_G(M, 64, true); else HZ_G(M, 64, false);      \
+    } else if (block == 256) {                                      \
+        if (multi) HZ_G(M, 256, true); else HZ_G(M, 256, false);    \
+    } else {                                                        \
+        if (multi) HZ_G(M, 128, true); else HZ_G(M, 128, false);    \
+    }
+    if (mode == 0) { HZ_GB(0) } else { HZ_GB(1) }
+#undef HZ_GB
+#undef HZ_G
+}
+
+int launch_trace_batch(const BatchDomain* doms, int ndom, int total, bool multi,
+                       const WalkParams& p, int mode, int block, int refill, int chunk,
+                       TraceState* ts, double* spill, hipStream_t st, std::string& err) {
+    if (refill < 1 || refill > 64) refill = 40;
+    if (chunk < 1) chunk = kChunk;
+    hipLaunchKernelGGL(k_clear_state, dim3(1), dim3(256), 0, st, ts);
+    if (mode == 0)
+        batch_block<0>(block, multi, doms, ndom, total, p, refill, chunk, ts, spill, st);
+    else if (mode == 1)
+        batch_block<1>(block, multi, doms, ndom, total, p, refill, chunk, ts, spill, st);
+    else {
+        err = "the learned search is not available in batches";
+        return -1;
+    }
+    HZ_HIP(hipGetLastError(), err);
+    return 0;
 }
 
 int launch_trace(const DeviceBvh& bvh, const CellFrame* cells, const int* order,

@@ -101,7 +101,7 @@ def horizon_svf_comp_py(clon, clat, hsurf, vlon, vlat, cells_of_vertex, num_hori
                         grid_type, dist_search, ray_org_elev, refine_factor, svf_type, *,
                         search_mode=None, leaf_size=None, block_size=None,
                         collect_counters=None, trace_kernel=None, refill=None,
-                        bvh_morton=None, stats=None):
+                        bvh_morton=None, return_slope_aspect=False, stats=None):
     """Compute the terrain horizon [deg] and sky view factor [-].
 
     Same contract as the reference binding (horizon_svf.pyx:15-124):
@@ -109,6 +109,8 @@ def horizon_svf_comp_py(clon, clat, hsurf, vlon, vlat, cells_of_vertex, num_hori
     (6, num_vertex) with 0-based cell indices and -2 fill.  Returns
     ``horizon`` (num_hori, ncell) and ``skyview`` (ncell,), float64.
     ``stats``: optional dict that receives the library statistics.
+    ``return_slope_aspect=True`` appends the terrain slope and aspect [deg]
+    per cell (an extension; the reference returns only the two arrays).
     """
     (num_hori, grid_type, dist_search, ray_org_elev, refine_factor,
      svf_type) = validate(clon, clat, hsurf, vlon, vlat, cells_of_vertex, num_hori,
@@ -121,6 +123,18 @@ def horizon_svf_comp_py(clon, clat, hsurf, vlon, vlat, cells_of_vertex, num_hori
     opt = default_options(search_mode=search_mode, leaf_size=leaf_size,
                           block_size=block_size, collect_counters=collect_counters,
                           trace_kernel=trace_kernel, refill=refill, bvh_morton=bvh_morton)
+    if return_slope_aspect:
+        with ResidentDomain(clon, clat, hsurf, vlon, vlat, cells_of_vertex, grid_type,
+                            ray_org_elev) as dom:
+            kw = {f: getattr(opt, f) for f in ("search_mode", "leaf_size", "block_size",
+                                               "collect_counters", "trace_kernel", "refill",
+                                               "bvh_morton")}
+            st_run = dom.run(num_hori, dist_search, refine_factor, svf_type, **kw)
+            horizon, skyview = dom.fetch()
+            slope, aspect = dom.slope_aspect()
+        if stats is not None:
+            stats.update(st_run)
+        return horizon, skyview, slope, aspect
     st = HzStats()
     rc = _lib.lib().horizon_svf_comp_ex(
         _ptr(clon, ctypes.c_double), _ptr(clat, ctypes.c_double),
@@ -186,6 +200,15 @@ class ResidentDomain:
         check(_lib.lib().hz_domain_descriptors(self._h, out.ctypes.data_as(
             ctypes.POINTER(ctypes.c_uint16))))
         return out
+
+    def slope_aspect(self):
+        """Terrain slope and aspect [deg] per cell (aspect: downhill azimuth,
+        clockwise from north)."""
+        slope = np.empty(self.ncell, np.float64)
+        aspect = np.empty(self.ncell, np.float64)
+        check(_lib.lib().hz_domain_slope_aspect(self._h, _ptr(slope, ctypes.c_double),
+                                                _ptr(aspect, ctypes.c_double)))
+        return slope, aspect
 
     def predictions(self):
         """Predicted horizon [ncell, 24] (deg) of the last search_mode=2 run."""

@@ -308,4 +308,25 @@ def tilted_plane(size_km: float, edge_km: float, slope_east: float,
     return dom
 
 
+def config4(n_domains: int = 64, seed: int = 404, cells_min: float = 5e3,
+            cells_max: float = 4e5):
+    """Batched uneven domains: cell counts log-uniform in [cells_min,
+    cells_max], edge lengths in {2, 1, 0.5} km, nhori in {24, 48}, each
+    domain from its own seed with the config-1 field recipe.  Returns a list
+    of (Domain, nhori)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n_domains):
+        n = float(np.exp(rng.uniform(np.log(cells_min), np.log(cells_max))))
+        edge = float(rng.choice([2.0, 1.0, 0.5]))
+        nhori = int(rng.choice([24, 48]))
+        side = math.sqrt(n * math.sqrt(3.0) / 4.0 * edge * edge)  # km, square box
+        s = seed * 1000 + k
+        dom = lattice_domain(side, side, edge, lon0_deg=8.5 + 0.01 * k, lat0_deg=46.5, seed=s)
+        dom.hsurf = terrain_grf(dom, beta=3.0, std=500.0, seed=s)
+        dom.meta.update(name=f"c4_{k}", seed=s, nhori=nhori)
+        out.append((dom, nhori))
+    return out
+
+
 CONFIGS = {"c1": config1, "c2": config2}

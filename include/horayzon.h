@@ -142,6 +142,17 @@ int hz_scene_export(const double* clon, const double* clat, const double* hsurf,
                     double* lon0_lat0);
 
 /*
+ * Terrain slope and aspect [deg] per cell (no counterpart in the reference,
+ * whose sky view factor is for a horizontal plane,
+ * mo_lradtopo_horayzon.cpp:320-332): the surface normal is the area-weighted
+ * normal of the mesh triangles around the cell's circumcentre; slope is its
+ * angle to the local vertical, aspect the azimuth (clockwise from north,
+ * [0, 360)) of its horizontal part, i.e. the downhill direction; 0 / 0 for a
+ * level cell or a cell that is no triangle corner.
+ */
+int hz_domain_slope_aspect(hz_domain* dom, double* slope, double* aspect);
+
+/*
  * Learned elevation bracket (no counterpart in the reference, which always
  * runs the full search, mo_lradtopo_horayzon.cpp:490-555).  A predictor is
  * a 192 -> 256 -> 256 -> 24 GELU network evaluated in BF16 on the matrix
@@ -183,6 +194,23 @@ int hz_trainer_fit(hz_trainer* t, int epochs, int batch, float lr, float* epoch_
 int hz_trainer_gradients(hz_trainer* t, const unsigned short* X, const float* T, int n,
                          float* grads, float* loss);
 int hz_trainer_predict(hz_trainer* t, const unsigned short* X, int n, float* out);
+
+/*
+ * Batched domains (no counterpart in the reference, which handles one domain
+ * per call): one persistent trace launch over the cells of many resident
+ * domains that share the search parameters.  Per run, every domain's LBVH
+ * and cell order are rebuilt (stream-ordered), then the trace launch runs --
+ * replayed from a captured HIP graph when use_graph is set (the graph is
+ * captured on the first run of a configuration).  Results stay in each
+ * domain; hz_domain_fetch reads them.  Domains must outlive the batch.
+ * Options: search_mode (walk or climb), leaf_size, block_size, refill,
+ * cell_order, bvh_morton, and reserved[0] = cells per queue fetch (0 = 64).
+ */
+typedef struct hz_batch hz_batch;
+int hz_batch_create(hz_domain* const* domains, int n, hz_batch** out);
+int hz_batch_run(hz_batch* b, int num_hori, double dist_search, int refine_factor, int svf_type,
+                 const hz_options* opt, int use_graph, hz_stats* stats);
+void hz_batch_destroy(hz_batch* b);
 
 /* Number of cells / mesh triangles of a resident domain. */
 int hz_domain_num_cell(const hz_domain* dom);
