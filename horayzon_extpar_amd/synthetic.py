@@ -308,6 +308,61 @@ def tilted_plane(size_km: float, edge_km: float, slope_east: float,
     return dom
 
 
+def _subset_lattice(xv, yv, tri, keep):
+    """Triangles tri[keep] with their vertices renumbered."""
+    t = tri[keep]
+    used = np.unique(t)
+    remap = np.full(xv.size, -1, np.int64)
+    remap[used] = np.arange(used.size)
+    return xv[used], yv[used], remap[t]
+
+
+def config3(seed: int = 303, core_km: float = 60.0, core_edge_km: float = 0.1,
+            ring_km: float = 40.0, ring_edge_km: float = 0.5) -> Domain:
+    """Fine-mesh stress: a core_km box at core_edge_km (~831k cells at the
+    defaults) inside an outer ring_km-wide ring at ring_edge_km (~148k cells)
+    that supplies the distant horizon.  Core and ring share no vertices: the
+    seam vertices have fewer than six cells (-2 fill), as where two ICON
+    grids meet.  Field: k^-2.5, std 900 m, plus 300 m cliff steps, shifted
+    to a minimum of 0."""
+    lon0, lat0 = math.radians(8.5), math.radians(46.5)
+    half = core_km * 500.0
+    xc, yc, tc = triangular_lattice(core_km * 1e3, core_km * 1e3, core_edge_km * 1e3)
+    outer = core_km + 2 * ring_km
+    xr, yr, tr = triangular_lattice(outer * 1e3, outer * 1e3, ring_edge_km * 1e3)
+    cxr = (xr[tr[:, 0]] + xr[tr[:, 1]] + xr[tr[:, 2]]) / 3.0
+    cyr = (yr[tr[:, 0]] + yr[tr[:, 1]] + yr[tr[:, 2]]) / 3.0
+    gap = ring_edge_km * 1e3
+    keep = (np.abs(cxr) > half + gap) | (np.abs(cyr) > half + gap)
+    xr, yr, tr = _subset_lattice(xr, yr, tr, keep)
+    xv = np.concatenate([xc, xr])
+    yv = np.concatenate([yc, yr])
+    tri = np.concatenate([tc, tr + xc.size])
+    rng = np.random.default_rng(seed)
+    uv = _plane_to_unit(xv, yv, lon0, lat0)
+    vlon, vlat = _unit_to_lonlat(uv)
+    a, b, c = uv[tri[:, 0]], uv[tri[:, 1]], uv[tri[:, 2]]
+    n = np.cross(b - a, c - a)
+    n /= np.linalg.norm(n, axis=1, keepdims=True)
+    n *= np.sign(np.sum(n * a, axis=1))[:, None]
+    clon, clat = _unit_to_lonlat(n)
+    cov = _cells_of_vertex(tri, xv.size, rng)
+    cx = (xv[tri[:, 0]] + xv[tri[:, 1]] + xv[tri[:, 2]]) / 3.0
+    cy = (yv[tri[:, 0]] + yv[tri[:, 1]] + yv[tri[:, 2]]) / 3.0
+    meta = dict(width_km=outer, height_km=outer, edge_km=ring_edge_km, core_km=core_km,
+                core_edge_km=core_edge_km, lon0_deg=8.5, lat0_deg=46.5, name="c3", seed=seed,
+                ncore=int(tc.shape[0]))
+    dom = Domain(clon=clon, clat=clat, hsurf=np.zeros_like(clon), vlon=vlon, vlat=vlat,
+                 cells_of_vertex=cov, meta=meta, cx=cx, cy=cy)
+    frng = np.random.default_rng(seed + 1)
+    sp = core_edge_km * 1e3 * 2.0
+    g, gx0, gy0 = _field_on_box(outer * 1e3, outer * 1e3, sp, 2.5, 900.0, frng)
+    z = _bilinear(g, gx0, gy0, sp, cx, cy)
+    z = add_cliffs(dom, z, 6, 300.0, seed + 2)
+    dom.hsurf = z - z.min()
+    return dom
+
+
 def config4(n_domains: int = 64, seed: int = 404, cells_min: float = 5e3,
             cells_max: float = 4e5):
     """Batched uneven domains: cell counts log-uniform in [cells_min,
