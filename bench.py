@@ -40,6 +40,8 @@ def main():
     ap.add_argument("--kernel", type=int, default=0, help="0 persistent, 1 one lane per cell")
     ap.add_argument("--refill", type=int, default=0)
     ap.add_argument("--morton", type=int, default=0, help="LBVH codes: 0 3D, 1 2D")
+    ap.add_argument("--width", type=int, default=0,
+                    help="traversal tree: 4 four-wide quantized (default), 2 binary")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-batch", action="store_true",
@@ -70,7 +72,8 @@ def main():
     print(f"[bench] workload {args.config}: ncell={dom_in.ncell} nvert={dom_in.nvert}",
           file=sys.stderr, flush=True)
     opts = dict(search_mode=mode, leaf_size=args.leaf or None, block_size=args.block or None,
-                trace_kernel=args.kernel, refill=args.refill or None, bvh_morton=args.morton)
+                trace_kernel=args.kernel, refill=args.refill or None, bvh_morton=args.morton,
+                bvh_width=args.width or None)
     with ResidentDomain(*dom_in.args(), grid_type=1, ray_org_elev=0.2) as dom:
         cst = dom.create_stats
         for _ in range(args.warmup):
@@ -118,8 +121,9 @@ def main():
                                f"{ncell} cells, {cnt['num_triangles']} triangles (grid_type 1)",
                    "num_hori": nhori, "dist_search_m": dist, "refine_factor": rf,
                    "svf_type": svf, "search": args.mode,
-                   "trace_kernel": ["persistent", "lane-per-cell", "persistent-4w",
-                                    "persistent-5w"][args.kernel]},
+                   "trace_kernel": ["persistent", "lane-per-cell"][args.kernel],
+                   "bvh": "binary" if (args.width == 2 or args.kernel == 1)
+                   else "four-wide, quantized child boxes"},
         "rays_per_s": rays / (tr * 1e-3),
         "rays_per_cell_sector": rays / (ncell * nhori),
         "ms_trace": tr, "ms_bvh": float(np.mean(t_bvh)), "ms_order": float(np.mean(t_ord)),
@@ -128,6 +132,9 @@ def main():
                      "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS, "traffic": None,
                      "kernel": "k_trace", "bytes_per_ray": b_ray,
                      "nodes_per_ray": n_node, "tris_per_ray": n_tri},
+        "trace_counters": {"refill_frac": cnt["refill_frac"],
+                           "lanes_per_iter": cnt["lanes_per_iter"],
+                           "bvh_nodes": cnt["bvh_nodes"]},
         "horizon_checksum": float(np.sum(h)), "skyview_mean": float(np.mean(s)),
     }
     if learned_out:

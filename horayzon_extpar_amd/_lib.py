@@ -32,7 +32,8 @@ class HzOptions(ctypes.Structure):
                 ("block_size", ctypes.c_int), ("collect_counters", ctypes.c_int),
                 ("cell_order", ctypes.c_int), ("verbose", ctypes.c_int),
                 ("trace_kernel", ctypes.c_int), ("refill", ctypes.c_int),
-                ("bvh_morton", ctypes.c_int), ("reserved", ctypes.c_int * 7)]
+                ("bvh_morton", ctypes.c_int), ("queue_chunk", ctypes.c_int),
+                ("bvh_width", ctypes.c_int), ("reserved", ctypes.c_int * 5)]
 
 
 class HzStats(ctypes.Structure):
@@ -45,7 +46,8 @@ class HzStats(ctypes.Structure):
                 ("t_trace_ms", ctypes.c_double), ("t_download_ms", ctypes.c_double),
                 ("t_total_ms", ctypes.c_double), ("lon0", ctypes.c_double),
                 ("lat0", ctypes.c_double), ("t_predict_ms", ctypes.c_double),
-                ("reserved", ctypes.c_double * 7)]
+                ("refill_frac", ctypes.c_double), ("lanes_per_iter", ctypes.c_double),
+                ("reserved", ctypes.c_double * 5)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}

@@ -31,9 +31,9 @@ class Batch:
 
     def run(self, num_hori=24, dist_search=40000.0, refine_factor=1, svf_type=2,
             use_graph=True, chunk=None, **opts):
-        opt = default_options(**{k: v for k, v in opts.items() if k != "chunk"})
+        opt = default_options(**opts)
         if chunk:
-            opt.reserved[0] = int(chunk)
+            opt.queue_chunk = int(chunk)
         st = HzStats()
         check(_lib.lib().hz_batch_run(self._h, int(num_hori), float(dist_search),
                                       int(refine_factor), int(svf_type), ctypes.byref(opt),

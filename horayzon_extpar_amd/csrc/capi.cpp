@@ -138,6 +138,8 @@ void hz_default_options(hz_options* o) {
     o->verbose = 0;
     o->trace_kernel = 0;
     o->refill = 40;
+    o->queue_chunk = 0;
+    o->bvh_width = 4;
 }
 
 void hz_domain_destroy(hz_domain* d) {
@@ -423,6 +425,9 @@ int hz_domain_run(hz_domain* d, int num_hori, double dist_search, int refine_fac
         return fail(HZ_EINVAL, "block_size must be 64, 128 or 256");
     int leaf = o.leaf_size ? o.leaf_size : 1;
     if (leaf < 1 || leaf > kMaxLeafSize) return fail(HZ_EINVAL, "leaf_size must be in [1, 8]");
+    int width = o.bvh_width ? o.bvh_width : 4;
+    if (width != 2 && width != 4) return fail(HZ_EINVAL, "bvh_width must be 2 or 4");
+    if (o.trace_kernel == 1) width = 2;  // the lane-per-cell kernel walks binary trees
     bool verbose = o.verbose || verbose_env();
     RCHK(hipSetDevice(d->device));
     double t0 = now_ms();
@@ -438,9 +443,9 @@ int hz_domain_run(hz_domain* d, int num_hori, double dist_search, int refine_fac
     d->bvh.nodes = d->d_nodes;
     d->bvh.tris = d->d_tris;
     if (o.bvh_morton < 0 || o.bvh_morton > 1) return fail(HZ_EINVAL, "bvh_morton must be 0 or 1");
-    rc = lbvh_build(d->work, d->d_vert, d->d_tri, (int)d->ntri, leaf, o.bvh_morton, st, d->bvh,
-                    o.collect_counters != 0, err);
-    if (rc) return fail(rc == -3 ? HZ_ENOMEM : HZ_EDEVICE, err);
+    rc = lbvh_build(d->work, d->d_vert, d->d_tri, (int)d->ntri, leaf, o.bvh_morton, width, st,
+                    d->bvh, o.collect_counters != 0, err);
+    if (rc) return fail(rc == -3 ? HZ_ENOMEM : (rc == -1 ? HZ_EINVAL : HZ_EDEVICE), err);
     RCHK(hipEventRecord(d->ev[1], st));
     const int* order = nullptr;
     if (o.cell_order) {
@@ -470,7 +475,12 @@ int hz_domain_run(hz_domain* d, int num_hori, double dist_search, int refine_fac
     RCHK(hipEventRecord(d->ev[3], st));
     TraceCounters c;
     RCHK(hipMemcpyAsync(&c, &d->d_state->cnt, sizeof(c), hipMemcpyDeviceToHost, st));
+    unsigned nodes4 = 0;
+    if (d->bvh.d_num_nodes)
+        RCHK(hipMemcpyAsync(&nodes4, d->bvh.d_num_nodes, sizeof(nodes4), hipMemcpyDeviceToHost,
+                            st));
     RCHK(hipStreamSynchronize(st));
+    if (d->bvh.d_num_nodes) d->bvh.num_nodes = (int)nodes4;
     float ms_bvh = 0, ms_ord = 0, ms_trace = 0;
     RCHK(hipEventElapsedTime(&ms_bvh, d->ev[0], d->ev[1]));
     RCHK(hipEventElapsedTime(&ms_ord, d->ev[1], d->ev[2]));
@@ -500,6 +510,9 @@ int hz_domain_run(hz_domain* d, int num_hori, double dist_search, int refine_fac
         stats->t_order_ms = ms_ord;
         stats->t_trace_ms = ms_trace;
         stats->t_predict_ms = ms_pred;
+        const double cyc = (double)c.cyc_refill + (double)c.cyc_trav;
+        stats->refill_frac = cyc > 0 ? (double)c.cyc_refill / cyc : 0.0;
+        stats->lanes_per_iter = c.trav_iters ? (double)c.nodes / (double)c.trav_iters : 0.0;
         stats->t_total_ms = t1 - t0;
         stats->lon0 = d->lon0;
         stats->lat0 = d->lat0;
@@ -609,8 +622,8 @@ struct hz_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     // configuration the graph was captured for
-    int g_mode = -1, g_block = 0, g_refill = 0, g_chunk = 0, g_multi = -1, g_nhori = 0, g_rf = 0,
-        g_svf = 0;
+    int g_mode = -1, g_block = 0, g_refill = 0, g_chunk = 0, g_multi = -1, g_width = 0,
+        g_nhori = 0, g_rf = 0, g_svf = 0;
     double g_dist = 0;
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
 };
@@ -702,8 +715,10 @@ int hz_batch_run(hz_batch* b, int num_hori, double dist_search, int refine_facto
     if (leaf < 1 || leaf > kMaxLeafSize) return fail(HZ_EINVAL, "leaf_size must be in [1, 8]");
     int refill = o.refill ? o.refill : 40;
     if (refill < 1 || refill > 64) return fail(HZ_EINVAL, "refill must be in [1, 64]");
-    int chunk = o.reserved[0] ? o.reserved[0] : 64;  // queue chunk (cells per fetch)
-    if (chunk < 1 || chunk > 4096) return fail(HZ_EINVAL, "chunk must be in [1, 4096]");
+    int chunk = o.queue_chunk ? o.queue_chunk : 64;  // cells per queue fetch
+    if (chunk < 1 || chunk > 4096) return fail(HZ_EINVAL, "queue_chunk must be in [1, 4096]");
+    int width = o.bvh_width ? o.bvh_width : 4;
+    if (width != 2 && width != 4) return fail(HZ_EINVAL, "bvh_width must be 2 or 4");
     RCHK(hipSetDevice(b->device));
     hipStream_t st = b->st;
     std::string err;
@@ -721,9 +736,9 @@ int hz_batch_run(hz_batch* b, int num_hori, double dist_search, int refine_facto
         }
         d->bvh.nodes = d->d_nodes;
         d->bvh.tris = d->d_tris;
-        rc = lbvh_build(d->work, d->d_vert, d->d_tri, (int)d->ntri, leaf, o.bvh_morton, st, d->bvh,
-                        false, err);
-        if (rc) return fail(rc == -3 ? HZ_ENOMEM : HZ_EDEVICE, err);
+        rc = lbvh_build(d->work, d->d_vert, d->d_tri, (int)d->ntri, leaf, o.bvh_morton, width, st,
+                        d->bvh, false, err);
+        if (rc) return fail(rc == -3 ? HZ_ENOMEM : (rc == -1 ? HZ_EINVAL : HZ_EDEVICE), err);
         if (o.cell_order) {
             rc = order_cells(d->work, d->d_cells, d->ncell, d->d_order, st, err);
             if (rc) return fail(HZ_EDEVICE, err);
@@ -745,7 +760,7 @@ int hz_batch_run(hz_batch* b, int num_hori, double dist_search, int refine_facto
                         hipMemcpyHostToDevice, st));
     const bool multi = leaf > 1;
     const int mode = o.search_mode;
-    size_t need = (size_t)batch_grid(mode, block, multi, b->total) * S_FIELDS_HOST;
+    size_t need = (size_t)batch_grid(mode, block, multi, width, b->total) * S_FIELDS_HOST;
     if (need > b->spill_cap) {
         if (b->d_spill) RCHK(hipFree(b->d_spill));
         b->d_spill = nullptr;
@@ -765,6 +780,7 @@ int hz_batch_run(hz_batch* b, int num_hori, double dist_search, int refine_facto
     if (use_graph) {
         const bool same = b->exec && b->g_mode == mode && b->g_block == block &&
                           b->g_refill == refill && b->g_chunk == chunk && b->g_multi == (int)multi &&
+                          b->g_width == width &&
                           b->g_nhori == num_hori && b->g_rf == refine_factor &&
                           b->g_svf == svf_type && b->g_dist == dist_search;
         if (!same) {
@@ -775,7 +791,7 @@ int hz_batch_run(hz_batch* b, int num_hori, double dist_search, int refine_facto
                 b->graph = nullptr;
             }
             RCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-            rc = launch_trace_batch(b->d_table, (int)b->doms.size(), b->total, multi, p, mode, block,
+            rc = launch_trace_batch(b->d_table, (int)b->doms.size(), b->total, multi, width, p, mode, block,
                                     refill, chunk, b->d_state, b->d_spill, st, err);
             hipGraph_t g = nullptr;
             hipError_t ec = hipStreamEndCapture(st, &g);
@@ -790,6 +806,7 @@ int hz_batch_run(hz_batch* b, int num_hori, double dist_search, int refine_facto
             b->g_refill = refill;
             b->g_chunk = chunk;
             b->g_multi = multi;
+            b->g_width = width;
             b->g_nhori = num_hori;
             b->g_rf = refine_factor;
             b->g_svf = svf_type;
@@ -797,7 +814,7 @@ int hz_batch_run(hz_batch* b, int num_hori, double dist_search, int refine_facto
         }
         RCHK(hipGraphLaunch(b->exec, st));
     } else {
-        rc = launch_trace_batch(b->d_table, (int)b->doms.size(), b->total, multi, p, mode, block,
+        rc = launch_trace_batch(b->d_table, (int)b->doms.size(), b->total, multi, width, p, mode, block,
                                 refill, chunk, b->d_state, b->d_spill, st, err);
         if (rc) return fail(HZ_EDEVICE, err);
     }

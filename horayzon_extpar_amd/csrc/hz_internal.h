@@ -42,6 +42,34 @@ struct alignas(16) BvhNode {
 };
 static_assert(sizeof(BvhNode) == 128, "BvhNode layout");
 
+// Four-wide node, 128 bytes, built by collapsing the binary LBVH (every
+// binary interior node at even depth becomes one node whose children are its
+// grandchildren, or its children where those are leaves).  Child boxes are
+// quantized to 8 bits per plane relative to the node's own box and rounded
+// outward: plane = org + q * 2^(e - 127).
+//   org[3], meta   : FP32 origin; meta bytes 0..2 = biased exponents e of
+//                    x, y, z, byte 3 = number of children (1..4, packed first)
+//   q[6]           : lo x, hi x, lo y, hi y, lo z, hi z; byte k = child k
+//   ref[4]         : >= 0 interior node (bits 0..27), bit 30 set when that
+//                    node holds inline triangles; < 0 leaf, ~ref = code:
+//                    bit 30 set: inline triangle, slot = bit 0;
+//                    else first | ((count - 1) << 27), triangles in leaf order
+//   tri[18]        : inline slots 0 (tri[0..8]) and 1 (tri[9..17])
+// The first 64 bytes hold everything but the inline triangles (and the first
+// two floats of slot 0), so a node without them is read as 64 bytes.
+struct alignas(16) Bvh4Node {
+    float org[3];
+    unsigned meta;
+    unsigned q[6];
+    int ref[4];
+    float tri[18];
+};
+static_assert(sizeof(Bvh4Node) == 128, "Bvh4Node layout");
+constexpr int kB4InlineBit = 1 << 30;
+constexpr int kB4CountShift = 27;
+constexpr int kB4FirstMask = (1 << kB4CountShift) - 1;
+constexpr int kMaxTriangles4 = 1 << kB4CountShift;
+
 constexpr int kLeafFirstBits = 28;
 constexpr int kLeafFirstMask = (1 << kLeafFirstBits) - 1;
 constexpr int kWideBit = 1 << 30;
@@ -51,12 +79,14 @@ constexpr int kMaxLeafSize = 8;
 // 3 float4 per triangle {v0.x, v0.y, v0.z, v1.x}, {v1.y, v1.z, v2.x, v2.y},
 // {v2.z, id, 0, 0}
 struct DeviceBvh {
-    BvhNode* nodes = nullptr;
+    BvhNode* nodes = nullptr;  // Bvh4Node records when width == 4 (same 128-byte stride)
     float4* tris = nullptr;
     int num_tri = 0;
     int num_nodes = 0;  // interior nodes (>= 1 whenever num_tri >= 1)
     int depth = 0;      // deepest interior level, root = 0
     int leaf_size = 1;  // largest leaf (> 1: leaves beyond the inline triangles exist)
+    int width = 2;      // 2: BvhNode, 4: Bvh4Node
+    const unsigned* d_num_nodes = nullptr;  // device count of four-wide nodes
 };
 
 // Constants of the elevation search (reference :577-587, :651-668).
@@ -67,6 +97,7 @@ struct WalkParams {
     double elev_sin_2ha, elev_cos_2ha;
     double shift_sin, shift_cos;  // of -azim_shift
     float tfar;             // (float) dist_search
+    float tfar_box;         // slab-test far limit: tfar * 1.0001 + 1 m
     int num_hori;
     int refine_factor;
     int svf_type;
@@ -129,9 +160,11 @@ struct LbvhWork {
     int* parent = nullptr;  // leaves [0, n), interior [n, 2n-1)
     void* boxes = nullptr;  // range-union work: sorted leaf boxes, prefix, suffix, sparse table
     float* part = nullptr;  // per-workgroup centroid bounds
-    unsigned* bounds = nullptr;  // 6 order-preserving encoded floats + depth + spare
+    unsigned* bounds = nullptr;  // 6 order-preserving encoded floats, depth, spare, four-wide node count
     void* sort_tmp = nullptr;
     size_t sort_tmp_bytes = 0;
+    void* scan_tmp = nullptr;  // four-wide collapse: node numbering scan
+    size_t scan_tmp_bytes = 0;
 };
 
 int lbvh_reserve(LbvhWork& w, int n, std::string& err);
@@ -139,9 +172,11 @@ void lbvh_free(LbvhWork& w);
 // Builds the BVH of ntri triangles (vertex indices into vert, 3 floats per
 // vertex).  out.nodes / out.tris must hold max(1, ntri - 1) nodes and 3 * ntri
 // float4.  Enqueued on stream; depth is read back only when want_depth.
-// morton_mode 0: 3D codes in a cube of the largest extent, 1: 2D x/y codes
+// morton_mode 0: 3D codes in a cube of the largest extent, 1: 2D x/y codes;
+// width 2: binary BvhNode tree, 4: four-wide Bvh4Node tree (its node count
+// is left on the device in *out.d_num_nodes and num_nodes is set to -1)
 int lbvh_build(LbvhWork& w, const float* vert, const int* tri, int ntri, int leaf_size,
-               int morton_mode, hipStream_t stream, DeviceBvh& out, bool want_depth,
+               int morton_mode, int width, hipStream_t stream, DeviceBvh& out, bool want_depth,
                std::string& err);
 
 // Morton order of the cell ray origins (2D, x/y of the ENU frame).
@@ -158,6 +193,11 @@ struct TraceCounters {
     unsigned long long nodes;
     unsigned long long tris;
     unsigned long long error;
+    // counters mode, persistent kernel: shader clock cycles each wavefront
+    // spent refilling lanes and traversing, and traversal iterations
+    unsigned long long cyc_refill;
+    unsigned long long cyc_trav;
+    unsigned long long trav_iters;
 };
 
 // Device-side state of one trace launch: counters plus the work queue of
@@ -165,9 +205,10 @@ struct TraceCounters {
 constexpr int kQueueStride = 32;
 struct TraceState {
     TraceCounters cnt;
-    int pad[24];
+    int pad[18];
     int queue[8 * kQueueStride];
 };
+static_assert(sizeof(TraceCounters) + 18 * sizeof(int) == 128, "queue offset");
 
 // kernel: 0 = persistent lanes with per-XCD work queues (default),
 //         1 = one lane per cell, azimuths in lock step
@@ -175,7 +216,7 @@ struct TraceState {
 //         once fewer than `refill` of the 64 lanes hold a live ray
 // One domain of a batched launch (device table entry).
 struct BatchDomain {
-    const BvhNode* nodes;
+    const BvhNode* nodes;  // Bvh4Node records when the batch runs four-wide trees
     const float4* tris;
     const CellFrame* cells;
     const int* order;
@@ -189,12 +230,12 @@ struct BatchDomain {
 
 // One persistent launch over the cells of ndom domains sharing the search
 // parameters p (p.ncell unused); total = sum of their cells.  mode 0 or 1.
-int launch_trace_batch(const BatchDomain* doms, int ndom, int total, bool multi,
+int launch_trace_batch(const BatchDomain* doms, int ndom, int total, bool multi, int width,
                        const WalkParams& p, int mode, int block, int refill, int chunk,
                        TraceState* ts, double* spill, hipStream_t st, std::string& err);
 // lanes of the batched launch (also settles its cached occupancy query, which
 // must not run inside a stream capture)
-int batch_grid(int mode, int block, bool multi, int total);
+int batch_grid(int mode, int block, bool multi, int width, int total);
 
 // spill: per-lane FP64 search state of the persistent kernel, at least
 // trace_spill_doubles(...) doubles
@@ -203,7 +244,7 @@ int launch_trace(const DeviceBvh& bvh, const CellFrame* cells, const int* order,
                  bool counters, double* horizon, double* skyview, TraceState* state,
                  double* spill, hipStream_t stream, std::string& err);
 size_t trace_spill_doubles(int ncell);
-constexpr int S_FIELDS_HOST = 9;  // FP64 fields of the per-lane search spill
+constexpr int S_FIELDS_HOST = 13;  // 8-byte fields of the per-lane search spill
 
 }  // namespace hz
 

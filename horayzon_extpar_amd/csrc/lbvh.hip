@@ -393,6 +393,191 @@ __global__ __launch_bounds__(kBlock) void k_tris(const int* __restrict__ sorted,
     out[3 * k + 2] = make_float4(vert[3 * c + 2], __int_as_float(t), 0.0f, 0.0f);
 }
 
+// --- four-wide collapse -----------------------------------------------------
+//
+// Binary interior node i becomes a four-wide node when its depth is even and
+// it holds more than leaf_size triangles (the root always does).  Its
+// children are its binary children, each opened into its own two children
+// unless it is a leaf or holds at most leaf_size triangles (then it is one
+// leaf of the four-wide node).  An opened child's interior children sit at
+// even depth again, so every four-wide child is itself a four-wide node.
+
+__device__ __forceinline__ int leaf_count(const int* __restrict__ range, int c) {
+    return range[2 * c + 1] - range[2 * c] + 1;
+}
+
+__global__ __launch_bounds__(kBlock) void k_b4_flag(const int* __restrict__ parent,
+                                                   const int* __restrict__ range, int n,
+                                                   int leaf_size, int* __restrict__ flag) {
+    int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n - 1) return;
+    int d = 0;
+    int p = parent[n + i];
+    while (p >= 0 && d < 4096) {  // the Karras hierarchy has depth < 64 + log2(n)
+        d++;
+        p = parent[n + p];
+    }
+    flag[i] = (i == 0 || ((d & 1) == 0 && leaf_count(range, i) > leaf_size)) ? 1 : 0;
+}
+
+__global__ void k_b4_count(const int* __restrict__ flag, const int* __restrict__ idx, int m,
+                           unsigned* __restrict__ out) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *out = (unsigned)(idx[m - 1] + flag[m - 1]);
+}
+
+// Does the four-wide node made from binary node g hold a single-triangle leaf
+// (and therefore inline triangles)?
+__device__ __forceinline__ bool b4_has_single(const int* __restrict__ child,
+                                              const int* __restrict__ range, int g,
+                                              int leaf_size) {
+    for (int s = 0; s < 2; s++) {
+        const int c = child[2 * g + s];
+        if (c < 0) return true;
+        if (leaf_count(range, c) <= leaf_size) continue;
+        if (child[2 * c] < 0 || child[2 * c + 1] < 0) return true;
+    }
+    return false;
+}
+
+// Quantize one axis: origin lo, scale 2^e with 255 * 2^e >= (hi - lo); child
+// planes rounded outward (exact arithmetic in FP64: the products q * 2^e are
+// exact and the sums carry at most one FP64 rounding, far below the 1 m box
+// margin).
+__device__ __forceinline__ unsigned b4_axis(const float* clo, const float* chi, int nch, float lo,
+                                            float hi, unsigned& qlo, unsigned& qhi) {
+    const double ext = (double)hi - (double)lo;
+    const double need = ext * (1.0 + 0x1p-20) / 255.0;
+    int e = -100;
+    if (need > 0x1p-100) {
+        e = ilogb(need);
+        if (ldexp(1.0, e) < need) e++;
+    }
+    const double sc = ldexp(1.0, e);
+    qlo = 0u;
+    qhi = 0u;
+    for (int k = 0; k < 4; k++) {
+        unsigned ql = 255u, qh = 0u;  // empty slot
+        if (k < nch) {
+            double a = floor(((double)clo[k] - (double)lo) / sc);
+            a = fmin(fmax(a, 0.0), 255.0);
+            while (a > 0.0 && (double)lo + a * sc > (double)clo[k]) a -= 1.0;
+            double b = ceil(((double)chi[k] - (double)lo) / sc);
+            b = fmin(fmax(b, 0.0), 255.0);
+            while (b < 255.0 && (double)lo + b * sc < (double)chi[k]) b += 1.0;
+            ql = (unsigned)a;
+            qh = (unsigned)b;
+        }
+        qlo |= ql << (8 * k);
+        qhi |= qh << (8 * k);
+    }
+    return (unsigned)(e + 127);
+}
+
+__global__ __launch_bounds__(kBlock) void k_pack4(const int* __restrict__ sorted, int n,
+                                                 const float4* __restrict__ lo,
+                                                 const float4* __restrict__ hi,
+                                                 const int* __restrict__ child,
+                                                 const int* __restrict__ range, int leaf_size,
+                                                 const int* __restrict__ flag,
+                                                 const int* __restrict__ idx,
+                                                 const float* __restrict__ vert,
+                                                 const int* __restrict__ tri,
+                                                 Bvh4Node* __restrict__ nodes) {
+    int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n - 1 || !flag[i]) return;
+    // children: kind 0 = interior (binary index), 1 = single leaf (sorted
+    // leaf index), 2 = leaf range (binary index)
+    int kid[4], kind[4];
+    int nch = 0;
+    for (int s = 0; s < 2; s++) {
+        const int c = child[2 * i + s];
+        if (c < 0) {
+            kid[nch] = ~c;
+            kind[nch++] = 1;
+        } else if (leaf_count(range, c) <= leaf_size) {
+            kid[nch] = c;
+            kind[nch++] = 2;
+        } else {
+            for (int t = 0; t < 2; t++) {
+                const int g = child[2 * c + t];
+                if (g < 0) {
+                    kid[nch] = ~g;
+                    kind[nch++] = 1;
+                } else {
+                    kid[nch] = g;
+                    kind[nch++] = leaf_count(range, g) <= leaf_size ? 2 : 0;
+                }
+            }
+        }
+    }
+    float clo[3][4], chi[3][4];
+    float blo[3] = {3.0e38f, 3.0e38f, 3.0e38f}, bhi[3] = {-3.0e38f, -3.0e38f, -3.0e38f};
+    for (int k = 0; k < nch; k++) {
+        const int slot = kind[k] == 1 ? sorted[kid[k]] : n + kid[k];
+        const float4 l = lo[slot], h = hi[slot];
+        const float lv[3] = {l.x, l.y, l.z}, hv[3] = {h.x, h.y, h.z};
+        for (int a = 0; a < 3; a++) {
+            clo[a][k] = grow_lo(lv[a]);
+            chi[a][k] = grow_hi(hv[a]);
+            blo[a] = fminf(blo[a], clo[a][k]);
+            bhi[a] = fmaxf(bhi[a], chi[a][k]);
+        }
+    }
+    Bvh4Node nd;
+    for (int k = 0; k < 18; k++) nd.tri[k] = 0.0f;
+    unsigned meta = (unsigned)nch << 24;
+    for (int a = 0; a < 3; a++) {
+        nd.org[a] = blo[a];
+        meta |= b4_axis(clo[a], chi[a], nch, blo[a], bhi[a], nd.q[2 * a], nd.q[2 * a + 1]) << (8 * a);
+    }
+    nd.meta = meta;
+    int inl = 0;
+    for (int k = 0; k < 4; k++) {
+        int r = 0;
+        if (k < nch) {
+            if (kind[k] == 0) {
+                r = idx[kid[k]] | (b4_has_single(child, range, kid[k], leaf_size) ? kWideBit : 0);
+            } else if (kind[k] == 1) {
+                if (inl < 2) {
+                    put_tri(&nd.tri[9 * inl], vert, tri, sorted[kid[k]]);
+                    r = ~(kB4InlineBit | inl);
+                    inl++;
+                } else {
+                    r = ~kid[k];  // one triangle in the leaf-order array
+                }
+            } else {
+                const int first = range[2 * kid[k]];
+                r = ~(first | ((leaf_count(range, kid[k]) - 1) << kB4CountShift));
+            }
+        }
+        nd.ref[k] = r;
+    }
+    nodes[idx[i]] = nd;
+}
+
+// One triangle: the root holds it as its only child, inline.
+__global__ void k_pack4_single(const float4* __restrict__ lo, const float4* __restrict__ hi,
+                               const float* __restrict__ vert, const int* __restrict__ tri,
+                               Bvh4Node* __restrict__ nodes, unsigned* __restrict__ count) {
+    if (threadIdx.x != 0) return;
+    const float4 l = lo[0], h = hi[0];
+    const float lv[3] = {l.x, l.y, l.z}, hv[3] = {h.x, h.y, h.z};
+    Bvh4Node nd;
+    for (int k = 0; k < 18; k++) nd.tri[k] = 0.0f;
+    unsigned meta = 1u << 24;
+    for (int a = 0; a < 3; a++) {
+        float cl[4] = {grow_lo(lv[a]), 0.f, 0.f, 0.f}, ch[4] = {grow_hi(hv[a]), 0.f, 0.f, 0.f};
+        nd.org[a] = cl[0];
+        meta |= b4_axis(cl, ch, 1, cl[0], ch[0], nd.q[2 * a], nd.q[2 * a + 1]) << (8 * a);
+    }
+    nd.meta = meta;
+    nd.ref[0] = ~(kB4InlineBit | 0);
+    nd.ref[1] = nd.ref[2] = nd.ref[3] = 0;
+    put_tri(&nd.tri[0], vert, tri, 0);
+    nodes[0] = nd;
+    *count = 1u;
+}
+
 // Depth of every interior node's deepest leaf path: each leaf walks to the
 // root counting levels (diagnostics and stack sizing).
 __global__ __launch_bounds__(kBlock) void k_depth(const int* __restrict__ parent, int n,
@@ -489,25 +674,38 @@ int lbvh_reserve(LbvhWork& w, int n, std::string& err) {
            err);
     w.sort_tmp_bytes = tmp;
     HZ_HIP(hipMalloc(&w.sort_tmp, tmp), err);
+    size_t stmp = 0;
+    HZ_HIP(rocprim::exclusive_scan(nullptr, stmp, w.vals[0], reinterpret_cast<int*>(w.keys[0]), 0,
+                                   (size_t)cap, rocprim::plus<int>()),
+           err);
+    w.scan_tmp_bytes = stmp;
+    HZ_HIP(hipMalloc(&w.scan_tmp, stmp), err);
     w.capacity = cap;
     return 0;
 }
 
 void lbvh_free(LbvhWork& w) {
     void* p[] = {w.keys[0], w.keys[1], w.vals[0], w.vals[1], w.aabb_lo, w.aabb_hi, w.child,
-                 w.range, w.parent, w.boxes, w.part, w.bounds, w.sort_tmp};
+                 w.range, w.parent, w.boxes, w.part, w.bounds, w.sort_tmp, w.scan_tmp};
     for (void* q : p)
         if (q) (void)hipFree(q);
     w = LbvhWork();
 }
 
 int lbvh_build(LbvhWork& w, const float* vert, const int* tri, int n, int leaf_size,
-               int morton_mode, hipStream_t st, DeviceBvh& out, bool want_depth, std::string& err) {
+               int morton_mode, int width, hipStream_t st, DeviceBvh& out, bool want_depth,
+               std::string& err) {
     out.num_tri = n;
     out.num_nodes = 0;
     out.depth = 0;
     out.leaf_size = std::max(1, std::min(leaf_size, kMaxLeafSize));
+    out.width = width == 4 ? 4 : 2;
+    out.d_num_nodes = nullptr;
     if (n == 0) return 0;
+    if (out.width == 4 && n >= kMaxTriangles4) {
+        err = "four-wide trees hold fewer than 2^27 triangles";
+        return -1;
+    }
     if (lbvh_reserve(w, n, err)) return -3;
     if (leaf_size < 1) leaf_size = 1;
     if (leaf_size > kMaxLeafSize) leaf_size = kMaxLeafSize;
@@ -516,13 +714,20 @@ int lbvh_build(LbvhWork& w, const float* vert, const int* tri, int n, int leaf_s
                        w.aabb_hi, w.part);
     hipLaunchKernelGGL(k_reduce_bounds, dim3(1), dim3(kBlock), 0, st, w.part, grid(n), w.bounds);
     if (n == 1) {
-        hipLaunchKernelGGL(k_pack_single, dim3(1), dim3(64), 0, st, w.aabb_lo, w.aabb_hi, vert,
-                           tri, out.nodes);
+        if (out.width == 4) {
+            hipLaunchKernelGGL(k_pack4_single, dim3(1), dim3(64), 0, st, w.aabb_lo, w.aabb_hi,
+                               vert, tri, reinterpret_cast<Bvh4Node*>(out.nodes), w.bounds + 8);
+            out.num_nodes = -1;
+            out.d_num_nodes = w.bounds + 8;
+        } else {
+            hipLaunchKernelGGL(k_pack_single, dim3(1), dim3(64), 0, st, w.aabb_lo, w.aabb_hi,
+                               vert, tri, out.nodes);
+            out.num_nodes = 1;
+        }
         hipLaunchKernelGGL(k_morton, dim3(1), dim3(kBlock), 0, st, w.aabb_lo, w.aabb_hi, n,
                            w.bounds, morton_mode, w.keys[0], w.vals[0]);
         hipLaunchKernelGGL(k_tris, dim3(1), dim3(kBlock), 0, st, w.vals[0], n, vert, tri,
                            out.tris);
-        out.num_nodes = 1;
         HZ_HIP(hipGetLastError(), err);
         return 0;
     }
@@ -549,11 +754,29 @@ int lbvh_build(LbvhWork& w, const float* vert, const int* tri, int n, int leaf_s
         hipLaunchKernelGGL(k_node_boxes, dim3(grid(n - 1)), dim3(kBlock), 0, st, w.range, n, leaf,
                            pre, suf, stab, nb, w.aabb_lo, w.aabb_hi);
     }
-    hipLaunchKernelGGL(k_pack, dim3(grid(n - 1)), dim3(kBlock), 0, st, sorted, n, w.aabb_lo,
-                       w.aabb_hi, w.child, w.range, leaf_size, vert, tri, out.nodes);
+    if (out.width == 4) {
+        // the sort's input buffers are free now: flags in vals[0], numbering in keys[0]
+        int* flag = w.vals[0];
+        int* idx = reinterpret_cast<int*>(w.keys[0]);
+        hipLaunchKernelGGL(k_b4_flag, dim3(grid(n - 1)), dim3(kBlock), 0, st, w.parent, w.range,
+                           n, leaf_size, flag);
+        size_t stmp = w.scan_tmp_bytes;
+        HZ_HIP(rocprim::exclusive_scan(w.scan_tmp, stmp, flag, idx, 0, (size_t)(n - 1),
+                                       rocprim::plus<int>(), st),
+               err);
+        hipLaunchKernelGGL(k_b4_count, dim3(1), dim3(64), 0, st, flag, idx, n - 1, w.bounds + 8);
+        hipLaunchKernelGGL(k_pack4, dim3(grid(n - 1)), dim3(kBlock), 0, st, sorted, n, w.aabb_lo,
+                           w.aabb_hi, w.child, w.range, leaf_size, flag, idx, vert, tri,
+                           reinterpret_cast<Bvh4Node*>(out.nodes));
+        out.num_nodes = -1;
+        out.d_num_nodes = w.bounds + 8;
+    } else {
+        hipLaunchKernelGGL(k_pack, dim3(grid(n - 1)), dim3(kBlock), 0, st, sorted, n, w.aabb_lo,
+                           w.aabb_hi, w.child, w.range, leaf_size, vert, tri, out.nodes);
+        out.num_nodes = n - 1;
+    }
     hipLaunchKernelGGL(k_tris, dim3(grid(n)), dim3(kBlock), 0, st, sorted, n, vert, tri,
                        out.tris);
-    out.num_nodes = n - 1;
     if (want_depth) {
         hipLaunchKernelGGL(k_depth, dim3(grid(n)), dim3(kBlock), 0, st, w.parent, n,
                            w.bounds + 6);

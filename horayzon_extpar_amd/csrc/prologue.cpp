@@ -112,6 +112,7 @@ WalkParams make_walk_params(int num_hori, int refine_factor, int svf_type, doubl
         shift = -(deg2rad(360.0) / (2.0 * num_hori)) + (deg2rad(360.0) / (2.0 * (double)L));
     sin_cos(-shift, &p.shift_sin, &p.shift_cos);
     p.tfar = (float)dist_search;
+    p.tfar_box = p.tfar * 1.0001f + 1.0f;
     p.num_hori = num_hori;
     p.refine_factor = refine_factor;
     p.svf_type = svf_type;

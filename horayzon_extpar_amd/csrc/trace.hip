@@ -29,7 +29,10 @@ namespace {
 
 constexpr double kPi = 3.14159265358979323846;
 constexpr int kLdsStack = 32;
-constexpr int kStackMax = 128;  // > the LBVH depth bound (63 code bits + 32 index bits)
+// The Karras hierarchy is at most 63 code bits + 32 index bits = 95 levels
+// deep: a binary step pushes at most one entry per level, a four-wide step
+// (half as many levels) at most three, so 3 * 48 entries bound both.
+constexpr int kStackMax = 160;
 
 struct D3 {
     double x, y, z;
@@ -79,6 +82,14 @@ __device__ __forceinline__ double sin_horizon(const WalkParams& P, double h) {
     return P.sin_h[k];
 }
 
+// Lane index (0..63), recomputed where it is used: the volatile asm keeps the
+// compiler from holding it in a register across the traversal loop.
+__device__ __forceinline__ int lane_now() {
+    int l;
+    asm volatile("v_mbcnt_lo_u32_b32 %0, -1, 0\n\tv_mbcnt_hi_u32_b32 %0, -1, %0" : "=v"(l));
+    return l;
+}
+
 __device__ __forceinline__ float sel3(float a, float b, float c, int k) {
     return k == 0 ? a : (k == 1 ? b : c);
 }
@@ -95,7 +106,7 @@ struct Ray {
 };
 
 __device__ __forceinline__ Ray make_ray(float o0, float o1, float o2, float d0, float d1,
-                                        float d2, float tfar) {
+                                        float d2, float tfar, float tfar_box) {
     Ray r;
     float a0 = fabsf(d0), a1 = fabsf(d1), a2 = fabsf(d2);
     int kz = a1 > a0 ? 1 : 0;
@@ -131,8 +142,8 @@ __device__ __forceinline__ Ray make_ray(float o0, float o1, float o2, float d0, 
     r.oix = o0 * r.ix;
     r.oiy = o1 * r.iy;
     r.oiz = o2 * r.iz;
-    r.tfar = tfar;
-    r.tfar_box = tfar * 1.0001f + 1.0f;
+    r.tfar = tfar;  // uniform kernel arguments: scalar registers
+    r.tfar_box = tfar_box;
     return r;
 }
 
@@ -190,7 +201,7 @@ __device__ __forceinline__ bool leaf_hit(const Ray& r, const float4* __restrict_
 // test both child boxes, test the triangles of hit leaf children at once,
 // then descend or pop.  Returns 0 while the query is open, 1 on a hit, 2
 // when the stack empties without one.  The stack keeps its first kLdsStack
-// entries in LDS at stk[k * BLOCK] and the rest in the private array ovf.
+// entries in LDS at stk[k * BLOCK + lane] and the rest in the private array ovf.
 template <bool COUNT, int BLOCK, bool MULTI>
 __device__ __forceinline__ int trav_step(const BvhNode* __restrict__ nodes,
                                          const float4* __restrict__ tris, int* stk, int* ovf,
@@ -253,7 +264,7 @@ __device__ __forceinline__ int trav_step(const BvhNode* __restrict__ nodes,
             return 1;
         }
         if (sp < kLdsStack)
-            stk[sp * BLOCK] = farc;
+            stk[sp * BLOCK + lane_now()] = farc;
         else
             ovf[sp - kLdsStack] = farc;
         sp++;
@@ -265,8 +276,140 @@ __device__ __forceinline__ int trav_step(const BvhNode* __restrict__ nodes,
     } else {
         if (sp == 0) return 2;
         sp--;
-        node = sp < kLdsStack ? stk[sp * BLOCK] : ovf[sp - kLdsStack];
+        node = sp < kLdsStack ? stk[sp * BLOCK + lane_now()] : ovf[sp - kLdsStack];
     }
+    return 0;
+}
+
+// One step of the any-hit traversal of the four-wide tree: fetch node `node`
+// (64 bytes, or 128 with its inline triangles), test the quantized boxes of
+// its children, test the triangles of hit leaf children at once, continue
+// with the nearest hit interior child and push the others.  Same return
+// values and stack as trav_step.
+__device__ __forceinline__ float qbyte(unsigned q, int k) {
+    return (float)((q >> (8 * k)) & 0xffu);
+}
+
+template <bool COUNT, int BLOCK>
+__device__ __forceinline__ int trav_step4(const Bvh4Node* __restrict__ nodes,
+                                          const float4* __restrict__ tris, int* stk, int* ovf,
+                                          const Ray& r, int& node, int& sp, unsigned& nvisit,
+                                          unsigned& ntest, int& err) {
+    const float4* np = reinterpret_cast<const float4*>(nodes + (node & kLeafFirstMask));
+    const float4 a = np[0], b = np[1], c = np[2], d = np[3];
+    float4 e4 = make_float4(0.f, 0.f, 0.f, 0.f), f4 = e4, g4 = e4, h4 = e4;
+    if (node & kWideBit) {  // inline triangle(s): second half of the node
+        e4 = np[4];
+        f4 = np[5];
+        g4 = np[6];
+        h4 = np[7];
+    }
+    if (COUNT) nvisit++;
+    const unsigned meta = __float_as_uint(a.w);
+    const int nch = (int)(meta >> 24);
+    // the ray parameter of plane org + q * 2^e is q * (2^e / dir) + (org / dir - o / dir)
+    const float ax = __uint_as_float((meta & 0xffu) << 23) * r.ix;
+    const float ay = __uint_as_float(((meta >> 8) & 0xffu) << 23) * r.iy;
+    const float az = __uint_as_float(((meta >> 16) & 0xffu) << 23) * r.iz;
+    const float bx = __builtin_fmaf(a.x, r.ix, -r.oix);
+    const float by = __builtin_fmaf(a.y, r.iy, -r.oiy);
+    const float bz = __builtin_fmaf(a.z, r.iz, -r.oiz);
+    // entry / exit planes by the sign of the direction
+    const unsigned qlx = __float_as_uint(b.x), qhx = __float_as_uint(b.y);
+    const unsigned qly = __float_as_uint(b.z), qhy = __float_as_uint(b.w);
+    const unsigned qlz = __float_as_uint(c.x), qhz = __float_as_uint(c.y);
+    const unsigned nx = r.ix >= 0.0f ? qlx : qhx, fx = r.ix >= 0.0f ? qhx : qlx;
+    const unsigned ny = r.iy >= 0.0f ? qly : qhy, fy = r.iy >= 0.0f ? qhy : qly;
+    const unsigned nz = r.iz >= 0.0f ? qlz : qhz, fz = r.iz >= 0.0f ? qhz : qlz;
+    const int r0 = __float_as_int(c.z), r1 = __float_as_int(c.w);
+    const int r2 = __float_as_int(d.x), r3 = __float_as_int(d.y);
+    const unsigned leafm = (r0 < 0 ? 1u : 0u) | (r1 < 0 ? 2u : 0u) | (r2 < 0 ? 4u : 0u) |
+                           (r3 < 0 ? 8u : 0u);
+    // hit children, and the nearest hit interior child (entry distance)
+    unsigned hit = 0u;
+    int best = -1;
+    float tb = 3.0e38f;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const float t0 = fmaxf(fmaxf(__builtin_fmaf(qbyte(nx, k), ax, bx),
+                                     __builtin_fmaf(qbyte(ny, k), ay, by)),
+                               fmaxf(__builtin_fmaf(qbyte(nz, k), az, bz), -1.0f));
+        const float t1 = fminf(fminf(__builtin_fmaf(qbyte(fx, k), ax, bx),
+                                     __builtin_fmaf(qbyte(fy, k), ay, by)),
+                               fminf(__builtin_fmaf(qbyte(fz, k), az, bz), r.tfar_box));
+        if (k < nch && t0 <= t1) {
+            hit |= 1u << k;
+            if (!((leafm >> k) & 1u) && t0 < tb) {
+                tb = t0;
+                best = k;
+            }
+        }
+    }
+    unsigned lh = hit & leafm;
+    // hit leaf children: inline triangles (slot 0: d.zw e4 f4.xyz, slot 1:
+    // f4.w g4 h4) or triangles of the leaf-order array; one test site
+#pragma unroll 1
+    while (lh) {
+        const int k = __builtin_ctz(lh);
+        lh &= lh - 1u;
+        const int code = ~(k == 0 ? r0 : (k == 1 ? r1 : (k == 2 ? r2 : r3)));
+        const bool inl = (code & kB4InlineBit) != 0;
+        const int first = code & kB4FirstMask;
+        const int cnt = inl ? 1 : (code >> kB4CountShift) + 1;
+#pragma unroll 1
+        for (int j = 0; j < cnt; j++) {
+            float v0, v1, v2, v3, v4, v5, v6, v7, v8;
+            if (inl) {
+                const bool z = (code & 1) == 0;
+                v0 = z ? d.z : f4.w;
+                v1 = z ? d.w : g4.x;
+                v2 = z ? e4.x : g4.y;
+                v3 = z ? e4.y : g4.z;
+                v4 = z ? e4.z : g4.w;
+                v5 = z ? e4.w : h4.x;
+                v6 = z ? f4.x : h4.y;
+                v7 = z ? f4.y : h4.z;
+                v8 = z ? f4.z : h4.w;
+            } else {
+                const float4* t = tris + 3 * (first + j);
+                const float4 t0 = t[0], t1 = t[1], t2 = t[2];
+                v0 = t0.x;
+                v1 = t0.y;
+                v2 = t0.z;
+                v3 = t0.w;
+                v4 = t1.x;
+                v5 = t1.y;
+                v6 = t1.z;
+                v7 = t1.w;
+                v8 = t2.x;
+            }
+            if (COUNT) ntest++;
+            if (tri_hit(r, v0, v1, v2, v3, v4, v5, v6, v7, v8)) return 1;
+        }
+    }
+    const unsigned ih = hit & ~leafm;
+    if (!ih) {
+        if (sp == 0) return 2;
+        sp--;
+        node = sp < kLdsStack ? stk[sp * BLOCK + lane_now()] : ovf[sp - kLdsStack];
+        return 0;
+    }
+    // nearest hit interior child next, the others onto the stack
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        if (!((ih >> k) & 1u) || k == best) continue;
+        if (sp >= kStackMax) {
+            err |= 1;
+            return 1;
+        }
+        const int rk = k == 0 ? r0 : (k == 1 ? r1 : (k == 2 ? r2 : r3));
+        if (sp < kLdsStack)
+            stk[sp * BLOCK + lane_now()] = rk;
+        else
+            ovf[sp - kLdsStack] = rk;
+        sp++;
+    }
+    node = best == 0 ? r0 : (best == 1 ? r1 : (best == 2 ? r2 : r3));
     return 0;
 }
 
@@ -313,7 +456,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace(const BvhNode* __restrict__ nod
     unsigned rays = 0, nvisit = 0, ntest = 0;
     int err = 0;
     if (lb < nblocks && idx < P.ncell) {
-        int* stk = stk_all + threadIdx.x;
+        int* stk = stk_all + (__builtin_amdgcn_readfirstlane(threadIdx.x) & ~63);
         const int cell = order ? order[idx] : idx;
         const CellFrame f = cells[cell];
         const D3 n = {f.n[0], f.n[1], f.n[2]};
@@ -326,7 +469,8 @@ __global__ __launch_bounds__(BLOCK) void k_trace(const BvhNode* __restrict__ nod
         auto cast = [&](const D3& d) -> bool {
             rays++;
             if (!have) return false;
-            Ray r = make_ray(o0, o1, o2, (float)d.x, (float)d.y, (float)d.z, P.tfar);
+            Ray r = make_ray(o0, o1, o2, (float)d.x, (float)d.y, (float)d.z, P.tfar,
+                             P.tfar_box);
             return occluded<COUNT, BLOCK>(nodes, tris, stk, r, nvisit, ntest, err);
         };
 
@@ -444,13 +588,20 @@ __global__ void k_clear_state(TraceState* ts) {
 // Per-lane FP64 search state kept in global memory between queries (it is
 // needed only when a query ends), so that the traversal loop runs with the
 // registers of the traversal alone.  Field f of lane g is spill[f * lanes + g].
-enum { S_DIR = 0, S_AX = 3, S_E = 6, S_SECTOR = 7, S_SVF = 8, S_FIELDS = 9 };
+// Fields S_I0.. hold two ints each: {cell, phase}, {azimuth j, lattice index},
+// {sector step, sector index}, {batch domain, unused}.
+enum {
+    S_DIR = 0, S_AX = 3, S_E = 6, S_SECTOR = 7, S_SVF = 8,
+    S_I0 = 9, S_I1 = 10, S_I2 = 11, S_I3 = 12, S_FIELDS = 13
+};
 static_assert(S_FIELDS == S_FIELDS_HOST, "spill layout");
 
+// TREE: 0 binary tree with single-triangle leaves, 1 binary tree with larger
+// leaves, 2 four-wide tree (Bvh4Node records behind `nodes`).
 // BATCH: the cells of several domains (same search parameters) form one
 // index space; doms[d] covers global indices [doms[d].start, doms[d + 1].start)
 // and each lane follows its own cell's domain (scene, outputs).
-template <int MODE, bool COUNT, int BLOCK, bool MULTI, bool BATCH>
+template <int MODE, bool COUNT, int BLOCK, int TREE, bool BATCH>
 __global__ __launch_bounds__(BLOCK, 4) void k_trace_pt(const BvhNode* __restrict__ nodes,
                                                     const float4* __restrict__ tris, int num_tri,
                                                     const CellFrame* __restrict__ cells,
@@ -463,7 +614,8 @@ __global__ __launch_bounds__(BLOCK, 4) void k_trace_pt(const BvhNode* __restrict
                                                     const BatchDomain* __restrict__ doms, int ndom,
                                                     int total) {
     __shared__ int stk_all[kLdsStack * BLOCK];
-    int* stk = stk_all + threadIdx.x;
+    // the wave's part of the stack: stk[k * BLOCK + lane] (the base is uniform)
+    int* stk = stk_all + (__builtin_amdgcn_readfirstlane(threadIdx.x) & ~63);
     int ovf[kStackMax - kLdsStack];
     const int lane = threadIdx.x & 63;
     const unsigned long long lt_mask = (1ull << lane) - 1ull;
@@ -474,6 +626,7 @@ __global__ __launch_bounds__(BLOCK, 4) void k_trace_pt(const BvhNode* __restrict
     const size_t lanes = (size_t)gridDim.x * BLOCK;
     double* sv = spill + (size_t)blockIdx.x * BLOCK + threadIdx.x;
 #define SV(f) sv[(size_t)(f) * lanes]
+#define SI(f) (*reinterpret_cast<int2*>(&sv[(size_t)(f) * lanes]))
 
     // wave-uniform queue state
     int cur = blockIdx.x & 7;  // workgroups are dealt round-robin over the 8 XCDs
@@ -481,26 +634,23 @@ __global__ __launch_bounds__(BLOCK, 4) void k_trace_pt(const BvhNode* __restrict
     int qb = 0, qe = 0;
     bool exhausted = false;
 
-    // lane state kept in registers
+    // lane state kept in registers across the traversal (the rest of the
+    // search state lives in the spill area)
     bool has_cell = false, live = false, last_hit = false;
-    int cell = 0, phase = P_AZSTART, j = 0, ksec = 0, jsec = 0, kk = 0;
-    float o0 = 0.f, o1 = 0.f, o2 = 0.f;
-    Ray ray = make_ray(0.f, 0.f, 0.f, 1.f, 0.f, 0.f, P.tfar);
+    Ray ray = make_ray(0.f, 0.f, 0.f, 1.f, 0.f, 0.f, P.tfar, P.tfar_box);
     int node = kWideBit, sp = 0, steps = 0;
     unsigned rays = 0, nvisit = 0, ntest = 0;
     int err = 0;
-    // the lane's domain (batch) or the single domain
-    int dom = 0;
+    unsigned long long cyc_refill = 0, cyc_trav = 0, iters = 0;  // COUNT only (wave-uniform)
+    // the scene of the lane's domain (batch) or of the single domain
     const BvhNode* lnodes = nodes;
     const float4* ltris = tris;
     bool have = num_tri > 0;
-    const CellFrame* lcells = cells;
-    double* lhorizon = horizon;
-    double* lskyview = skyview;
-    int lncell = P.ncell;
 
     while (true) {
         // ---------------- refill ----------------
+        unsigned long long c0 = 0;
+        if (COUNT) c0 = __builtin_amdgcn_s_memtime();
         for (int pass = 0; pass < 4; pass++) {
             const bool want = !live && !has_cell;
             const unsigned long long need = __ballot(want);
@@ -527,6 +677,8 @@ __global__ __launch_bounds__(BLOCK, 4) void k_trace_pt(const BvhNode* __restrict
                     const int rank = __popcll(need & lt_mask);
                     if (want && rank < avail) {
                         const int g = qb + rank;
+                        int cell = 0, dom = 0;
+                        const CellFrame* lcells = cells;
                         if (BATCH) {
                             int lo = 0, hi = ndom - 1;  // last domain with start <= g
                             while (lo < hi) {
@@ -541,19 +693,13 @@ __global__ __launch_bounds__(BLOCK, 4) void k_trace_pt(const BvhNode* __restrict
                             ltris = bd.tris;
                             have = bd.num_tri > 0;
                             lcells = bd.cells;
-                            lhorizon = bd.horizon;
-                            lskyview = bd.skyview;
-                            lncell = bd.ncell;
                         } else {
                             cell = order ? order[g] : g;
                         }
                         has_cell = true;
-                        const CellFrame f = lcells[cell];
-                        const D3 n = {f.n[0], f.n[1], f.n[2]};
-                        const D3 north = {f.north[0], f.north[1], f.north[2]};
-                        o0 = f.org[0];
-                        o1 = f.org[1];
-                        o2 = f.org[2];
+                        const CellFrame* f = lcells + cell;
+                        const D3 n = {f->n[0], f->n[1], f->n[2]};
+                        const D3 north = {f->north[0], f->north[1], f->north[2]};
                         const D3 dir = rodrigues(north, n, P.shift_sin, P.shift_cos);
                         SV(S_DIR) = dir.x;
                         SV(S_DIR + 1) = dir.y;
@@ -561,18 +707,32 @@ __global__ __launch_bounds__(BLOCK, 4) void k_trace_pt(const BvhNode* __restrict
                         SV(S_E) = 0.0;
                         SV(S_SECTOR) = 0.0;
                         SV(S_SVF) = 0.0;
-                        j = 0;
-                        ksec = 0;
-                        jsec = 0;
-                        phase = P_AZSTART;
+                        SI(S_I0) = make_int2(cell, P_AZSTART);
+                        SI(S_I1) = make_int2(0, 0);
+                        SI(S_I2) = make_int2(0, 0);
+                        if (BATCH) SI(S_I3) = make_int2(dom, 0);
                     }
                     qb += min(__popcll(need), avail);
                 }
             }
             if (has_cell && !live) {
                 // consume the last answer and issue the next query of this cell
-                const double* fn = lcells[cell].n;
-                const D3 n = {fn[0], fn[1], fn[2]};
+                const int2 i0 = SI(S_I0), i1 = SI(S_I1), i2 = SI(S_I2);
+                const int cell = i0.x;
+                int phase = i0.y, j = i1.x, kk = i1.y, ksec = i2.x, jsec = i2.y;
+                const CellFrame* lcells = cells;
+                double* lhorizon = horizon;
+                double* lskyview = skyview;
+                int lncell = P.ncell;
+                if (BATCH) {
+                    const BatchDomain& bd = doms[SI(S_I3).x];
+                    lcells = bd.cells;
+                    lhorizon = bd.horizon;
+                    lskyview = bd.skyview;
+                    lncell = bd.ncell;
+                }
+                const CellFrame* fr = lcells + cell;
+                const D3 n = {fr->n[0], fr->n[1], fr->n[2]};
                 D3 dir = {SV(S_DIR), SV(S_DIR + 1), SV(S_DIR + 2)};
                 D3 ax = {0.0, 0.0, 0.0};
                 double e = 0.0;
@@ -697,7 +857,11 @@ __global__ __launch_bounds__(BLOCK, 4) void k_trace_pt(const BvhNode* __restrict
                 }
                 if (issue) {
                     rays++;
-                    ray = make_ray(o0, o1, o2, (float)q.x, (float)q.y, (float)q.z, P.tfar);
+                    ray = make_ray(fr->org[0], fr->org[1], fr->org[2], (float)q.x, (float)q.y,
+                                   (float)q.z, P.tfar, P.tfar_box);
+                    SI(S_I0) = make_int2(cell, phase);
+                    SI(S_I1) = make_int2(j, kk);
+                    SI(S_I2) = make_int2(ksec, jsec);
                     node = kWideBit;  // the root is read in full
                     sp = 0;
                     steps = 0;
@@ -721,13 +885,24 @@ __global__ __launch_bounds__(BLOCK, 4) void k_trace_pt(const BvhNode* __restrict
         __asm__ __volatile__("" ::: "memory");
         if (!__any(live)) break;
         // ---------------- traversal ----------------
+        unsigned long long c1 = 0;
+        if (COUNT) {
+            c1 = __builtin_amdgcn_s_memtime();
+            cyc_refill += c1 - c0;
+        }
         const int thresh = exhausted ? 1 : refill;
         while (true) {
+            if (COUNT) iters++;
             if (live) {
                 int s = 2;
                 if (have) {
-                    s = trav_step<COUNT, BLOCK, MULTI>(lnodes, ltris, stk, ovf, ray, node, sp, nvisit,
-                                                       ntest, err);
+                    if constexpr (TREE == 2)
+                        s = trav_step4<COUNT, BLOCK>(reinterpret_cast<const Bvh4Node*>(lnodes),
+                                                     ltris, stk, ovf, ray, node, sp, nvisit,
+                                                     ntest, err);
+                    else
+                        s = trav_step<COUNT, BLOCK, TREE == 1>(lnodes, ltris, stk, ovf, ray, node,
+                                                               sp, nvisit, ntest, err);
                     // a BVH of n triangles has n - 1 interior nodes; more steps than
                     // that in one query can only come from a malformed tree
                     if (s == 0 && ++steps > (1 << 28)) {
@@ -742,7 +917,9 @@ __global__ __launch_bounds__(BLOCK, 4) void k_trace_pt(const BvhNode* __restrict
             }
             if (__popcll(__ballot(live)) < thresh) break;
         }
+        if (COUNT) cyc_trav += __builtin_amdgcn_s_memtime() - c1;
     }
+#undef SI
 #undef SV
     const unsigned long long wr = wave_sum(rays);
     const unsigned long long wn = COUNT ? wave_sum(nvisit) : 0ull;
@@ -753,12 +930,15 @@ __global__ __launch_bounds__(BLOCK, 4) void k_trace_pt(const BvhNode* __restrict
         if (COUNT) {
             atomicAdd(&ts->cnt.nodes, wn);
             atomicAdd(&ts->cnt.tris, wt);
+            atomicAdd(&ts->cnt.cyc_refill, cyc_refill);
+            atomicAdd(&ts->cnt.cyc_trav, cyc_trav);
+            atomicAdd(&ts->cnt.trav_iters, iters);
         }
         if (werr) atomicOr(&ts->cnt.error, 1ull);
     }
 }
 
-template <int MODE, bool COUNT, int BLOCK, bool MULTI, bool BATCH>
+template <int MODE, bool COUNT, int BLOCK, int TREE, bool BATCH>
 int pt_grid(int ncell) {
     static const int occ_grid = []() {
         int dev = 0, cus = 0, per = 0;
@@ -766,9 +946,8 @@ int pt_grid(int ncell) {
         if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
             return 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                &per, reinterpret_cast<const void*>(&k_trace_pt<MODE, COUNT, BLOCK, MULTI, BATCH>),
-                BLOCK, 0) !=
-            hipSuccess)
+                &per, reinterpret_cast<const void*>(&k_trace_pt<MODE, COUNT, BLOCK, TREE, BATCH>),
+                BLOCK, 0) != hipSuccess)
             return 0;
         return std::max(1, per) * std::max(1, cus);
     }();
@@ -777,36 +956,42 @@ int pt_grid(int ncell) {
     return std::max(8, (g + 7) / 8 * 8);
 }
 
-template <int MODE, bool COUNT, int BLOCK, bool MULTI>
+template <int MODE, bool COUNT, int BLOCK, int TREE>
 void launch_pt(const DeviceBvh& bvh, const CellFrame* cells, const int* order,
                const WalkParams& p, int refill, double* horizon, double* skyview,
                TraceState* ts, double* spill, hipStream_t st) {
-    hipLaunchKernelGGL((k_trace_pt<MODE, COUNT, BLOCK, MULTI, false>),
-                       dim3(pt_grid<MODE, COUNT, BLOCK, MULTI, false>(p.ncell)), dim3(BLOCK), 0,
+    hipLaunchKernelGGL((k_trace_pt<MODE, COUNT, BLOCK, TREE, false>),
+                       dim3(pt_grid<MODE, COUNT, BLOCK, TREE, false>(p.ncell)), dim3(BLOCK), 0,
                        st, bvh.nodes, bvh.tris, bvh.num_tri, cells, order, p, refill, kChunk,
                        horizon, skyview, ts, spill, nullptr, 0, 0);
 }
 
-template <int MODE, int BLOCK, bool MULTI>
+template <int MODE, int BLOCK, int TREE>
 void launch_pt_batch(const BatchDomain* doms, int ndom, int total, const WalkParams& p,
                      int refill, int chunk, TraceState* ts, double* spill, hipStream_t st) {
-    hipLaunchKernelGGL((k_trace_pt<MODE, false, BLOCK, MULTI, true>),
-                       dim3(pt_grid<MODE, false, BLOCK, MULTI, true>(total)), dim3(BLOCK), 0, st,
+    hipLaunchKernelGGL((k_trace_pt<MODE, false, BLOCK, TREE, true>),
+                       dim3(pt_grid<MODE, false, BLOCK, TREE, true>(total)), dim3(BLOCK), 0, st,
                        nullptr, nullptr, 0, nullptr, nullptr, p, refill, chunk, nullptr, nullptr,
                        ts, spill, doms, ndom, total);
 }
+
+inline int tree_kind(bool multi, int width) { return width == 4 ? 2 : (multi ? 1 : 0); }
 
 template <int MODE, bool COUNT, int BLOCK>
 void launch_one(const DeviceBvh& bvh, const CellFrame* cells, const int* order,
                 const WalkParams& p, int kernel, bool multi, int refill, double* horizon,
                 double* skyview, TraceState* ts, double* spill, hipStream_t st) {
     if (kernel == 0) {
-        if (multi)
-            launch_pt<MODE, COUNT, BLOCK, true>(bvh, cells, order, p, refill, horizon, skyview, ts,
-                                                spill, st);
+        const int tk = tree_kind(multi, bvh.width);
+        if (tk == 2)
+            launch_pt<MODE, COUNT, BLOCK, 2>(bvh, cells, order, p, refill, horizon, skyview, ts,
+                                             spill, st);
+        else if (tk == 1)
+            launch_pt<MODE, COUNT, BLOCK, 1>(bvh, cells, order, p, refill, horizon, skyview, ts,
+                                             spill, st);
         else
-            launch_pt<MODE, COUNT, BLOCK, false>(bvh, cells, order, p, refill, horizon, skyview,
-                                                 ts, spill, st);
+            launch_pt<MODE, COUNT, BLOCK, 0>(bvh, cells, order, p, refill, horizon, skyview, ts,
+                                             spill, st);
         return;
     }
     if constexpr (MODE == 2) {
@@ -834,19 +1019,41 @@ void launch_block(int block, const DeviceBvh& bvh, const CellFrame* cells, const
                                      skyview, ts, spill, st);
 }
 
+template <int MODE, int BLOCK>
+void batch_tree(int tk, const BatchDomain* doms, int ndom, int total, const WalkParams& p,
+                int refill, int chunk, TraceState* ts, double* spill, hipStream_t st) {
+    if (tk == 2)
+        launch_pt_batch<MODE, BLOCK, 2>(doms, ndom, total, p, refill, chunk, ts, spill, st);
+    else if (tk == 1)
+        launch_pt_batch<MODE, BLOCK, 1>(doms, ndom, total, p, refill, chunk, ts, spill, st);
+    else
+        launch_pt_batch<MODE, BLOCK, 0>(doms, ndom, total, p, refill, chunk, ts, spill, st);
+}
+
 template <int MODE>
-void batch_block(int block, bool multi, const BatchDomain* doms, int ndom, int total,
+void batch_block(int block, int tk, const BatchDomain* doms, int ndom, int total,
                  const WalkParams& p, int refill, int chunk, TraceState* ts, double* spill,
                  hipStream_t st) {
-#define HZ_B(BL, MU) launch_pt_batch<MODE, BL, MU>(doms, ndom, total, p, refill, chunk, ts, spill, st)
-    if (block == 64) {
-        if (multi) HZ_B(64, true); else HZ_B(64, false);
-    } else if (block == 256) {
-        if (multi) HZ_B(256, true); else HZ_B(256, false);
-    } else {
-        if (multi) HZ_B(128, true); else HZ_B(128, false);
-    }
-#undef HZ_B
+    if (block == 64)
+        batch_tree<MODE, 64>(tk, doms, ndom, total, p, refill, chunk, ts, spill, st);
+    else if (block == 256)
+        batch_tree<MODE, 256>(tk, doms, ndom, total, p, refill, chunk, ts, spill, st);
+    else
+        batch_tree<MODE, 128>(tk, doms, ndom, total, p, refill, chunk, ts, spill, st);
+}
+
+template <int MODE, int BLOCK>
+int batch_grid_tree(int tk, int total) {
+    if (tk == 2) return pt_grid<MODE, false, BLOCK, 2, true>(total) * BLOCK;
+    if (tk == 1) return pt_grid<MODE, false, BLOCK, 1, true>(total) * BLOCK;
+    return pt_grid<MODE, false, BLOCK, 0, true>(total) * BLOCK;
+}
+
+template <int MODE>
+int batch_grid_block(int block, int tk, int total) {
+    if (block == 64) return batch_grid_tree<MODE, 64>(tk, total);
+    if (block == 256) return batch_grid_tree<MODE, 256>(tk, total);
+    return batch_grid_tree<MODE, 128>(tk, total);
 }
 
 }  // namespace
@@ -861,31 +1068,22 @@ size_t trace_spill_doubles(int ncell) {
     return lanes * S_FIELDS;
 }
 
-int batch_grid(int mode, int block, bool multi, int total) {
-#define HZ_G(M, BL, MU) return pt_grid<M, false, BL, MU, true>(total) * BL
-#define HZ_GB(M)                                                    \
-    if (block == 64) {                                              \
-        if (multi) HZ_G(M, 64, true); else HZ_G(M, 64, false);      \
-    } else if (block == 256) {                                      \
-        if (multi) HZ_G(M, 256, true); else HZ_G(M, 256, false);    \
-    } else {                                                        \
-        if (multi) HZ_G(M, 128, true); else HZ_G(M, 128, false);    \
-    }
-    if (mode == 0) { HZ_GB(0) } else { HZ_GB(1) }
-#undef HZ_GB
-#undef HZ_G
+int batch_grid(int mode, int block, bool multi, int width, int total) {
+    const int tk = tree_kind(multi, width);
+    return mode == 0 ? batch_grid_block<0>(block, tk, total) : batch_grid_block<1>(block, tk, total);
 }
 
-int launch_trace_batch(const BatchDomain* doms, int ndom, int total, bool multi,
+int launch_trace_batch(const BatchDomain* doms, int ndom, int total, bool multi, int width,
                        const WalkParams& p, int mode, int block, int refill, int chunk,
                        TraceState* ts, double* spill, hipStream_t st, std::string& err) {
     if (refill < 1 || refill > 64) refill = 40;
     if (chunk < 1) chunk = kChunk;
+    const int tk = tree_kind(multi, width);
     hipLaunchKernelGGL(k_clear_state, dim3(1), dim3(256), 0, st, ts);
     if (mode == 0)
-        batch_block<0>(block, multi, doms, ndom, total, p, refill, chunk, ts, spill, st);
+        batch_block<0>(block, tk, doms, ndom, total, p, refill, chunk, ts, spill, st);
     else if (mode == 1)
-        batch_block<1>(block, multi, doms, ndom, total, p, refill, chunk, ts, spill, st);
+        batch_block<1>(block, tk, doms, ndom, total, p, refill, chunk, ts, spill, st);
     else {
         err = "the learned search is not available in batches";
         return -1;
@@ -903,6 +1101,10 @@ int launch_trace(const DeviceBvh& bvh, const CellFrame* cells, const int* order,
     hipLaunchKernelGGL(k_clear_state, dim3(1), dim3(256), 0, st, ts);
     if (mode == 2 && (kernel == 1 || !p.pred)) {
         err = "the learned search needs the persistent kernel and predictions";
+        return -1;
+    }
+    if (kernel == 1 && bvh.width != 2) {
+        err = "the lane-per-cell kernel traverses binary trees only";
         return -1;
     }
 #define HZ_LAUNCH(M)                                                                          \

@@ -44,7 +44,11 @@ typedef struct hz_options {
     int refill;           /* persistent kernel: refill lanes once fewer than this many of
                              the 64 lanes of a wavefront hold a live query (1..64; 0 = default) */
     int bvh_morton;       /* LBVH codes: 0 = 3D in a cube of the largest extent, 1 = 2D x/y */
-    int reserved[7];
+    int queue_chunk;      /* persistent kernel: cells taken per work-queue fetch (0 = 64) */
+    int bvh_width;        /* traversal tree: 4 = four-wide nodes with quantized child boxes
+                             (default), 2 = binary nodes with full-precision boxes; the
+                             lane-per-cell kernel always uses 2 */
+    int reserved[5];
 } hz_options;
 
 typedef struct hz_stats {
@@ -65,7 +69,11 @@ typedef struct hz_stats {
     double t_total_ms;           /* wall time of the call */
     double lon0, lat0;           /* ENU origin [rad] */
     double t_predict_ms;         /* device raster + descriptors + predictor (learned mode) */
-    double reserved[7];
+    double refill_frac;          /* collect_counters, persistent kernel: share of wavefront
+                                    time spent refilling lanes (search state, next query) */
+    double lanes_per_iter;       /* collect_counters, persistent kernel: live lanes (of 64) per
+                                    traversal iteration */
+    double reserved[5];
 } hz_stats;
 
 /*
@@ -204,7 +212,7 @@ int hz_trainer_predict(hz_trainer* t, const unsigned short* X, int n, float* out
  * captured on the first run of a configuration).  Results stay in each
  * domain; hz_domain_fetch reads them.  Domains must outlive the batch.
  * Options: search_mode (walk or climb), leaf_size, block_size, refill,
- * cell_order, bvh_morton, and reserved[0] = cells per queue fetch (0 = 64).
+ * cell_order, bvh_morton, bvh_width and queue_chunk.
  */
 typedef struct hz_batch hz_batch;
 int hz_batch_create(hz_domain* const* domains, int n, hz_batch** out);

@@ -68,14 +68,35 @@ def test_kernel_variants_bit_exact(c1, kernel, block, refill):
     assert np.max(np.abs(s - so)) <= SVF_TOL
 
 
-@pytest.mark.parametrize("leaf,morton", [(2, 0), (8, 0), (1, 1), (4, 1)])
-def test_tree_independent(c1, leaf, morton):
+@pytest.mark.parametrize("leaf,morton,width",
+                         [(2, 0, 4), (8, 0, 4), (1, 1, 4), (4, 1, 4), (1, 0, 2), (2, 0, 2),
+                          (8, 1, 2)])
+def test_tree_independent(c1, leaf, morton, width):
     # the occlusion answer is the OR over all triangles: any tree gives the same bits
-    print(f"case c1 leaf={leaf} morton={morton}", flush=True)
+    print(f"case c1 leaf={leaf} morton={morton} width={width}", flush=True)
     h4, s4 = horizon_svf_comp_py(*c1.args(), 24, 1, 40000.0, 0.2, 1, 2)
     h, s = horizon_svf_comp_py(*c1.args(), 24, 1, 40000.0, 0.2, 1, 2, leaf_size=leaf,
-                               block_size=64, bvh_morton=morton)
+                               block_size=64, bvh_morton=morton, bvh_width=width)
     assert np.array_equal(h, h4) and np.array_equal(s, s4)
+
+
+def test_c2_binary_and_four_wide_trees_agree(c2):
+    # full-size check of the quantized four-wide boxes: every one of the 13M
+    # angles and the query count equal those of the full-precision binary tree
+    print("case c2 width 2 vs 4", flush=True)
+    with ResidentDomain(*c2.args(), grid_type=1, ray_org_elev=0.2) as dom:
+        st4 = dom.run(24, 40000.0, 1, 2, bvh_width=4, collect_counters=1)
+        h4, s4 = dom.fetch()
+        st2 = dom.run(24, 40000.0, 1, 2, bvh_width=2, collect_counters=1)
+        h2, s2 = dom.fetch()
+    print(f"c2 nodes/ray: binary {st2['node_visits'] / st2['num_rays']:.1f} "
+          f"four-wide {st4['node_visits'] / st4['num_rays']:.1f}; tris/ray "
+          f"{st2['tri_tests'] / st2['num_rays']:.1f} / {st4['tri_tests'] / st4['num_rays']:.1f}; "
+          f"nodes {st2['bvh_nodes']} / {st4['bvh_nodes']}", flush=True)
+    assert np.array_equal(h4, h2) and np.array_equal(s4, s2)
+    assert st4["num_rays"] == st2["num_rays"]
+    assert st4["kernel_error"] == 0 and st2["kernel_error"] == 0
+    assert 0 < st4["bvh_nodes"] < st2["bvh_nodes"]
 
 
 def test_flat_plain():
