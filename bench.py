@@ -41,7 +41,7 @@ def main():
     ap.add_argument("--refill", type=int, default=0)
     ap.add_argument("--morton", type=int, default=0, help="LBVH codes: 0 3D, 1 2D")
     ap.add_argument("--width", type=int, default=0,
-                    help="traversal tree: 4 four-wide quantized (default), 2 binary")
+                    help="traversal tree: 2 binary (default), 4 four-wide quantized")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-batch", action="store_true",
@@ -122,8 +122,9 @@ def main():
                    "num_hori": nhori, "dist_search_m": dist, "refine_factor": rf,
                    "svf_type": svf, "search": args.mode,
                    "trace_kernel": ["persistent", "lane-per-cell"][args.kernel],
-                   "bvh": "binary" if (args.width == 2 or args.kernel == 1)
-                   else "four-wide, quantized child boxes"},
+                   "bvh": "four-wide, quantized child boxes" if (args.width == 4 and
+                                                                 args.kernel == 0)
+                   else "binary"},
         "rays_per_s": rays / (tr * 1e-3),
         "rays_per_cell_sector": rays / (ncell * nhori),
         "ms_trace": tr, "ms_bvh": float(np.mean(t_bvh)), "ms_order": float(np.mean(t_ord)),

@@ -63,6 +63,10 @@ SIGNATURES = [
      _ARR6 + [c_double_p, c_double_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
               ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int,
               ctypes.POINTER(HzOptions), ctypes.POINTER(HzStats)]),
+    ("horizon_svf_comp_fortran", None,
+     _ARR6 + [c_double_p, c_double_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+              ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_char_p,
+              c_int_p]),
     ("hz_last_error", ctypes.c_char_p, []),
     ("hz_default_options", None, [ctypes.POINTER(HzOptions)]),
     ("hz_domain_create", ctypes.c_int,

@@ -124,6 +124,55 @@ struct hz_domain {
     bool ran = false;
 };
 
+// The progress report the reference prints to stdout (mo_lradtopo_horayzon.cpp
+// :240-315, :427, :589-680, :765-776), as text: mesh, ENU origin, SVF form,
+// BVH build, tracing and the ray statistics (rays per cell and output sector,
+// :770), or the failure.
+static std::string run_report(const hz_stats& s, int num_cell, int num_hori, int grid_type,
+                              int svf_type, int rc) {
+    const double r2d = 180.0 / 3.14159265358979323846;
+    char line[512];
+    std::string out;
+    const char* rule =
+        "-------------------------------------------------------------------------------\n";
+    out += rule;
+    out += "Horizon and SVF computation with HIP on gfx950\n";
+    out += rule;
+    if (rc != 0 && s.num_triangles == 0) {
+        std::snprintf(line, sizeof(line), "horizon_svf_comp failed (%d): %s\n", rc, g_err.c_str());
+        return out + line;
+    }
+    out += grid_type == 0 ? "Triangle mesh from ICON grid cell circumcenters only\n"
+                          : "Triangle mesh from ICON grid cell circumcenters and vertices\n";
+    std::snprintf(line, sizeof(line),
+                  "Number of triangles in mesh: %lld\n"
+                  "Triangle mesh building and coordinate transforms: %.2f s\n"
+                  "Origin of ENU coordinate system: %.4f deg lat, %.4f deg lon\n",
+                  s.num_triangles, s.t_prologue_ms * 1e-3, s.lat0 * r2d, s.lon0 * r2d);
+    out += line;
+    std::snprintf(line, sizeof(line), "Sky view factor: %s\n",
+                  svf_type == 0   ? "pure geometric SVF"
+                  : svf_type == 1 ? "geometric scaled with sin(horizon)"
+                                  : "geometric scaled with sin(horizon)**2");
+    out += line;
+    std::snprintf(line, sizeof(line),
+                  "Building bounding volume hierarchy (BVH): %.2f s\n"
+                  "Ray tracing: %.2f s\n"
+                  "Number of rays shot: %lld\n"
+                  "Average number of rays per cell and azimuth sector: %.2f\n",
+                  s.t_bvh_ms * 1e-3, s.t_trace_ms * 1e-3, s.num_rays,
+                  num_cell > 0 && num_hori > 0 ? (double)s.num_rays / ((double)num_cell * num_hori)
+                                               : 0.0);
+    out += line;
+    if (rc != 0) {
+        std::snprintf(line, sizeof(line), "horizon_svf_comp failed (%d): %s\n", rc,
+                      g_err.c_str());
+        out += line;
+    }
+    out += rule;
+    return out;
+}
+
 extern "C" {
 
 const char* hz_last_error(void) { return g_err.c_str(); }
@@ -139,7 +188,7 @@ void hz_default_options(hz_options* o) {
     o->trace_kernel = 0;
     o->refill = 40;
     o->queue_chunk = 0;
-    o->bvh_width = 4;
+    o->bvh_width = 2;
 }
 
 void hz_domain_destroy(hz_domain* d) {
@@ -425,7 +474,7 @@ int hz_domain_run(hz_domain* d, int num_hori, double dist_search, int refine_fac
         return fail(HZ_EINVAL, "block_size must be 64, 128 or 256");
     int leaf = o.leaf_size ? o.leaf_size : 1;
     if (leaf < 1 || leaf > kMaxLeafSize) return fail(HZ_EINVAL, "leaf_size must be in [1, 8]");
-    int width = o.bvh_width ? o.bvh_width : 4;
+    int width = o.bvh_width ? o.bvh_width : 2;
     if (width != 2 && width != 4) return fail(HZ_EINVAL, "bvh_width must be 2 or 4");
     if (o.trace_kernel == 1) width = 2;  // the lane-per-cell kernel walks binary trees
     bool verbose = o.verbose || verbose_env();
@@ -604,6 +653,24 @@ void horizon_svf_comp(double* clon, double* clat, double* hsurf, double* vlon, d
     if (rc) std::fprintf(stderr, "horizon_svf_comp failed (%d): %s\n", rc, g_err.c_str());
 }
 
+void horizon_svf_comp_fortran(double* clon, double* clat, double* hsurf, double* vlon,
+                              double* vlat, int* cells_of_vertex, double* horizon,
+                              double* skyview, int num_cell, int num_vertex, int num_hori,
+                              int grid_type, double dist_search, double ray_org_elev,
+                              int refine_factor, int svf_type, char* buffer, int* buffer_len) {
+    hz_stats st;
+    std::memset(&st, 0, sizeof(st));
+    int rc = horizon_svf_comp_ex(clon, clat, hsurf, vlon, vlat, cells_of_vertex, horizon, skyview,
+                                 num_cell, num_vertex, num_hori, grid_type, dist_search,
+                                 ray_org_elev, refine_factor, svf_type, nullptr, &st);
+    const std::string text = run_report(st, num_cell, num_hori, grid_type, svf_type, rc);
+    if (buffer_len) {
+        const int n = buffer && *buffer_len > 0 ? std::min(*buffer_len, (int)text.size()) : 0;
+        if (n > 0) std::memcpy(buffer, text.data(), (size_t)n);
+        *buffer_len = n;
+    }
+}
+
 }  // extern "C"
 
 // ---------------------------------------------------------------------------
@@ -717,7 +784,7 @@ int hz_batch_run(hz_batch* b, int num_hori, double dist_search, int refine_facto
     if (refill < 1 || refill > 64) return fail(HZ_EINVAL, "refill must be in [1, 64]");
     int chunk = o.queue_chunk ? o.queue_chunk : 64;  // cells per queue fetch
     if (chunk < 1 || chunk > 4096) return fail(HZ_EINVAL, "queue_chunk must be in [1, 4096]");
-    int width = o.bvh_width ? o.bvh_width : 4;
+    int width = o.bvh_width ? o.bvh_width : 2;
     if (width != 2 && width != 4) return fail(HZ_EINVAL, "bvh_width must be 2 or 4");
     RCHK(hipSetDevice(b->device));
     hipStream_t st = b->st;

@@ -45,9 +45,10 @@ typedef struct hz_options {
                              the 64 lanes of a wavefront hold a live query (1..64; 0 = default) */
     int bvh_morton;       /* LBVH codes: 0 = 3D in a cube of the largest extent, 1 = 2D x/y */
     int queue_chunk;      /* persistent kernel: cells taken per work-queue fetch (0 = 64) */
-    int bvh_width;        /* traversal tree: 4 = four-wide nodes with quantized child boxes
-                             (default), 2 = binary nodes with full-precision boxes; the
-                             lane-per-cell kernel always uses 2 */
+    int bvh_width;        /* traversal tree: 2 = binary nodes with full-precision boxes
+                             (default), 4 = four-wide nodes with 8-bit quantized child boxes
+                             (half the node fetches, same results); the lane-per-cell kernel
+                             always uses 2 */
     int reserved[5];
 } hz_options;
 
@@ -101,6 +102,22 @@ int horizon_svf_comp_ex(const double* clon, const double* clat, const double* hs
                         int num_hori, int grid_type, double dist_search, double ray_org_elev,
                         int refine_factor, int svf_type, const hz_options* opt,
                         hz_stats* stats);
+
+/*
+ * Fortran-interface variant of horizon_svf_comp: the reference's commented
+ * EXTPAR interface (mo_lradtopo_horayzon.cpp:561-574, :778-786) adds
+ * `char* buffer, int* buffer_len` and copies what it printed to stdout into
+ * the caller's buffer.  Here the same report (mesh, ENU origin, SVF form, BVH
+ * and tracing times, rays per cell and sector, or the failure) is written to
+ * buffer: at most *buffer_len bytes, not NUL-terminated (a Fortran
+ * CHARACTER), and *buffer_len is set to the number of bytes written.
+ * Outputs are NaN on failure, as for horizon_svf_comp.
+ */
+void horizon_svf_comp_fortran(double* clon, double* clat, double* hsurf, double* vlon,
+                              double* vlat, int* cells_of_vertex, double* horizon,
+                              double* skyview, int num_cell, int num_vertex, int num_hori,
+                              int grid_type, double dist_search, double ray_org_elev,
+                              int refine_factor, int svf_type, char* buffer, int* buffer_len);
 
 /* Message of the calling thread's last failure ("" if none). */
 const char* hz_last_error(void);

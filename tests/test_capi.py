@@ -173,3 +173,30 @@ def test_c_abi_no_device_fails_loudly():
     assert np.all(np.isnan(h)) and np.all(np.isnan(s))
     with pytest.raises(_lib.HzError):
         horizon_svf_comp_py(*_args(d))
+
+
+def _fortran_call(d, num_hori=24, buf_len=4096):
+    L = _lib.lib()
+    h = np.empty((num_hori, d.ncell))
+    s = np.empty(d.ncell)
+    buf = ctypes.create_string_buffer(buf_len)
+    n = ctypes.c_int(buf_len)
+    p = lambda a, t: a.ctypes.data_as(ctypes.POINTER(t))  # noqa: E731
+    cov = np.ascontiguousarray(d.cells_of_vertex, np.int32)
+    L.horizon_svf_comp_fortran(p(d.clon, ctypes.c_double), p(d.clat, ctypes.c_double),
+                               p(d.hsurf, ctypes.c_double), p(d.vlon, ctypes.c_double),
+                               p(d.vlat, ctypes.c_double), p(cov, ctypes.c_int),
+                               p(h, ctypes.c_double), p(s, ctypes.c_double), d.ncell, d.nvert,
+                               num_hori, 1, 40000.0, 0.2, 1, 2, buf, ctypes.byref(n))
+    return h, s, buf.raw[:n.value].decode()
+
+
+def test_fortran_variant_reports_into_buffer():
+    # reference :561-574 / :778-786: the report goes to the caller's buffer,
+    # truncated to buffer_len and not NUL-terminated
+    d = synthetic.lattice_domain(10.0, 10.0, 2.0)
+    h, s, text = _fortran_call(d, num_hori=3)
+    assert np.all(np.isnan(h)) and np.all(np.isnan(s))
+    assert "failed" in text and "num_hori" in text
+    _, _, short = _fortran_call(d, num_hori=3, buf_len=20)
+    assert len(short) == 20 and text.startswith(short)

@@ -132,3 +132,29 @@ def test_c2_sampled_and_invariants(c2):
     _report("c2 sampled", h[:, m], ho[:, m])
     assert np.array_equal(h[:, m], ho[:, m])
     assert np.max(np.abs(s[m] - so[m])) <= SVF_TOL
+
+
+def test_fortran_variant_matches(c1):
+    # the EXTPAR-interface variant computes the same arrays and reports the
+    # reference's ray statistics into the caller's buffer
+    import ctypes
+    from horayzon_extpar_amd import _lib
+    print("case c1 fortran variant", flush=True)
+    st = {}
+    h0, s0 = horizon_svf_comp_py(*c1.args(), 24, 1, 40000.0, 0.2, 1, 2, stats=st)
+    h = np.empty_like(h0)
+    s = np.empty_like(s0)
+    buf = ctypes.create_string_buffer(4096)
+    n = ctypes.c_int(4096)
+    p = lambda a, t: a.ctypes.data_as(ctypes.POINTER(t))  # noqa: E731
+    cov = np.ascontiguousarray(c1.cells_of_vertex, np.int32)
+    _lib.lib().horizon_svf_comp_fortran(
+        p(c1.clon, ctypes.c_double), p(c1.clat, ctypes.c_double), p(c1.hsurf, ctypes.c_double),
+        p(c1.vlon, ctypes.c_double), p(c1.vlat, ctypes.c_double), p(cov, ctypes.c_int),
+        p(h, ctypes.c_double), p(s, ctypes.c_double), c1.ncell, c1.nvert, 24, 1, 40000.0, 0.2,
+        1, 2, buf, ctypes.byref(n))
+    text = buf.raw[:n.value].decode()
+    print(text, flush=True)
+    assert np.array_equal(h, h0) and np.array_equal(s, s0)
+    assert f"Number of rays shot: {st['num_rays']}" in text
+    assert f"Number of triangles in mesh: {st['num_triangles']}" in text
