@@ -104,6 +104,7 @@ def main():
     b_ray = 64.0 * n_node + 36.0 * n_tri
     bytes_launch = rays * b_ray + ncell * (64 + 8 * (nhori + 1))
     achieved = bytes_launch / (tr * 1e-3) / 1e9
+    traffic = profiled_traffic(args)
     out = {
         "metric": "horizon+SVF cells/s @nhori=24",
         "value": value,
@@ -130,7 +131,9 @@ def main():
         "ms_trace": tr, "ms_bvh": float(np.mean(t_bvh)), "ms_order": float(np.mean(t_ord)),
         "ms_prologue_host": cst.t_prologue_ms, "ms_upload": cst.t_upload_ms,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS,
-                     "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
+                     "traffic": traffic[0] if traffic else None,
+                     "traffic_source": traffic[1] if traffic else None,
                      "kernel": "k_trace", "bytes_per_ray": b_ray,
                      "nodes_per_ray": n_node, "tris_per_ray": n_tri},
         "trace_counters": {"refill_frac": cnt["refill_frac"],
@@ -250,6 +253,24 @@ def calibrate_copy(mib):
     print(f"[bench] calibration copy: {mib} MiB read + {mib} MiB written", file=sys.stderr,
           flush=True)
     del a, b
+
+
+def profiled_traffic(args):
+    """(HBM bytes per launch, source) of the default walk kernel on C2, from
+    the committed rocprofv3 PMC summary (FETCH_SIZE and WRITE_SIZE, scaled by
+    a 1 GiB copy); None for any other configuration."""
+    default = (args.config == "c2" and args.mode == "walk" and args.kernel == 0
+               and args.width in (0, 2) and args.leaf in (0, 1) and args.block in (0, 128)
+               and args.morton == 0)
+    path = os.path.join(ROOT, "profiles", "k_trace_c2_pmc.md")
+    if not default or not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return (int(line.split()[1]),
+                    "profiles/k_trace_c2_pmc.md: rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 "
+                    "(1 GiB copy calibration), " + line.split(None, 2)[2].strip())
+    return None
 
 
 def border_affected(dom_in, num_hori, dist_search, margin_m=3000.0):
