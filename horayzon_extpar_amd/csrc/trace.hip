@@ -33,6 +33,7 @@ constexpr int kLdsStack = 32;
 // deep: a binary step pushes at most one entry per level, a four-wide step
 // (half as many levels) at most three, so 3 * 48 entries bound both.
 constexpr int kStackMax = 160;
+constexpr int kHintRounds = 8;  // queries answered by the last occluder per refill
 
 struct D3 {
     double x, y, z;
@@ -184,7 +185,7 @@ __device__ __forceinline__ bool tri_hit(const Ray& r, float v0x, float v0y, floa
 
 template <bool COUNT>
 __device__ __forceinline__ bool leaf_hit(const Ray& r, const float4* __restrict__ tris, int ref,
-                                         unsigned& ntest) {
+                                         unsigned& ntest, int& hit_leaf) {
     int code = ~ref;
     int first = code & kLeafFirstMask;
     int cnt = (code >> kLeafFirstBits) + 1;
@@ -192,7 +193,10 @@ __device__ __forceinline__ bool leaf_hit(const Ray& r, const float4* __restrict_
         const float4* t = tris + 3 * (first + k);
         float4 t0 = t[0], t1 = t[1], t2 = t[2];
         if (COUNT) ntest++;
-        if (tri_hit(r, t0.x, t0.y, t0.z, t0.w, t1.x, t1.y, t1.z, t1.w, t2.x)) return true;
+        if (tri_hit(r, t0.x, t0.y, t0.z, t0.w, t1.x, t1.y, t1.z, t1.w, t2.x)) {
+            hit_leaf = first + k;
+            return true;
+        }
     }
     return false;
 }
@@ -206,7 +210,7 @@ template <bool COUNT, int BLOCK, bool MULTI>
 __device__ __forceinline__ int trav_step(const BvhNode* __restrict__ nodes,
                                          const float4* __restrict__ tris, int* stk, int* ovf,
                                          const Ray& r, int& node, int& sp, unsigned& nvisit,
-                                         unsigned& ntest, int& err) {
+                                         unsigned& ntest, int& err, int& hit_leaf) {
     const float4* np = reinterpret_cast<const float4*>(nodes + (node & kLeafFirstMask));
     const float4 a = np[0], b = np[1], c = np[2], d = np[3];
     float4 e4 = make_float4(0.f, 0.f, 0.f, 0.f), f4 = e4, g4 = e4, h4 = e4;
@@ -242,15 +246,17 @@ __device__ __forceinline__ int trav_step(const BvhNode* __restrict__ nodes,
             if (!act) continue;
             const int ref = s == 0 ? r0 : r1;
             if (MULTI && ref != ~(~ref & kLeafFirstMask)) {
-                if (leaf_hit<COUNT>(r, tris, ref, ntest)) return 1;
+                if (leaf_hit<COUNT>(r, tris, ref, ntest, hit_leaf)) return 1;
                 continue;
             }
             if (COUNT) ntest++;
             const bool z = s == 0;
             if (tri_hit(r, z ? d.z : f4.w, z ? d.w : g4.x, z ? e4.x : g4.y, z ? e4.y : g4.z,
                         z ? e4.z : g4.w, z ? e4.w : h4.x, z ? f4.x : h4.y, z ? f4.y : h4.z,
-                        z ? f4.z : h4.w))
+                        z ? f4.z : h4.w)) {
+                hit_leaf = ~ref;  // a single-triangle leaf: its leaf-order index
                 return 1;
+            }
         }
         h0 = h0 && r0 >= 0;
         h1 = h1 && r1 >= 0;
@@ -428,8 +434,9 @@ __device__ __forceinline__ bool occluded(const BvhNode* __restrict__ nodes,
             err |= 4;
             return true;
         }
+        int hit_leaf = 0;
         int s = trav_step<COUNT, BLOCK, true>(nodes, tris, stk, ovf, r, node, sp, nvisit, ntest,
-                                              err);
+                                              err, hit_leaf);
         if (s) return s == 1;
     }
 }
@@ -637,6 +644,7 @@ __global__ __launch_bounds__(BLOCK, 4) void k_trace_pt(const BvhNode* __restrict
     // lane state kept in registers across the traversal (the rest of the
     // search state lives in the spill area)
     bool has_cell = false, live = false, last_hit = false;
+    int hint = -1;  // leaf-order index of the triangle that occluded the cell's last hit
     Ray ray = make_ray(0.f, 0.f, 0.f, 1.f, 0.f, 0.f, P.tfar, P.tfar_box);
     int node = kWideBit, sp = 0, steps = 0;
     unsigned rays = 0, nvisit = 0, ntest = 0;
@@ -697,6 +705,7 @@ __global__ __launch_bounds__(BLOCK, 4) void k_trace_pt(const BvhNode* __restrict
                             cell = order ? order[g] : g;
                         }
                         has_cell = true;
+                        hint = -1;
                         const CellFrame* f = lcells + cell;
                         const D3 n = {f->n[0], f->n[1], f->n[2]};
                         const D3 north = {f->north[0], f->north[1], f->north[2]};
@@ -743,6 +752,20 @@ __global__ __launch_bounds__(BLOCK, 4) void k_trace_pt(const BvhNode* __restrict
                 double sector = SV(S_SECTOR), svf = SV(S_SVF);
                 D3 q = dir;  // direction of the query to issue
                 bool issue = false;
+                // the last occluder, fetched now so that the load overlaps the search
+                // state update below
+                float4 ht0 = make_float4(0.f, 0.f, 0.f, 0.f), ht1 = ht0, ht2 = ht0;
+                if (hint >= 0) {
+                    const float4* t = ltris + 3 * (size_t)hint;
+                    ht0 = t[0];
+                    ht1 = t[1];
+                    ht2 = t[2];
+                }
+                // a query that the cell's last occluding triangle already blocks is
+                // answered here, without a traversal (the answer is the OR over all
+                // triangles either way); a few in a row, then the tree decides
+                for (int hint_round = 0;; hint_round++) {
+                issue = false;
                 while (true) {
                     if (phase == P_AZSTART) {
                         phase = P_FIRST;
@@ -855,10 +878,21 @@ __global__ __launch_bounds__(BLOCK, 4) void k_trace_pt(const BvhNode* __restrict
                     }
                     phase = P_AZSTART;
                 }
+                if (!issue) break;
+                rays++;
+                ray = make_ray(fr->org[0], fr->org[1], fr->org[2], (float)q.x, (float)q.y,
+                               (float)q.z, P.tfar, P.tfar_box);
+                if (hint >= 0 && hint_round < kHintRounds) {
+                    if (COUNT) ntest++;
+                    if (tri_hit(ray, ht0.x, ht0.y, ht0.z, ht0.w, ht1.x, ht1.y, ht1.z, ht1.w,
+                                ht2.x)) {
+                        last_hit = true;
+                        continue;
+                    }
+                }
+                break;
+                }
                 if (issue) {
-                    rays++;
-                    ray = make_ray(fr->org[0], fr->org[1], fr->org[2], (float)q.x, (float)q.y,
-                                   (float)q.z, P.tfar, P.tfar_box);
                     SI(S_I0) = make_int2(cell, phase);
                     SI(S_I1) = make_int2(j, kk);
                     SI(S_I2) = make_int2(ksec, jsec);
@@ -902,7 +936,7 @@ __global__ __launch_bounds__(BLOCK, 4) void k_trace_pt(const BvhNode* __restrict
                                                      ntest, err);
                     else
                         s = trav_step<COUNT, BLOCK, TREE == 1>(lnodes, ltris, stk, ovf, ray, node,
-                                                               sp, nvisit, ntest, err);
+                                                               sp, nvisit, ntest, err, hint);
                     // a BVH of n triangles has n - 1 interior nodes; more steps than
                     // that in one query can only come from a malformed tree
                     if (s == 0 && ++steps > (1 << 28)) {
