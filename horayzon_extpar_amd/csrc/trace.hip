@@ -137,9 +137,12 @@ __device__ __forceinline__ Ray make_ray(float o0, float o1, float o2, float d0, 
     float e0 = fabsf(d0) < 1e-20f ? copysignf(1e-20f, d0) : d0;
     float e1 = fabsf(d1) < 1e-20f ? copysignf(1e-20f, d1) : d1;
     float e2 = fabsf(d2) < 1e-20f ? copysignf(1e-20f, d2) : d2;
-    r.ix = 1.0f / e0;
-    r.iy = 1.0f / e1;
-    r.iz = 1.0f / e2;
+    // slab-test reciprocals: the hardware approximation (1 ulp) is enough, the
+    // boxes carry a margin of at least 1 m; the watertight test's Sx, Sy, Sz
+    // above are IEEE divisions
+    r.ix = __builtin_amdgcn_rcpf(e0);
+    r.iy = __builtin_amdgcn_rcpf(e1);
+    r.iz = __builtin_amdgcn_rcpf(e2);
     r.oix = o0 * r.ix;
     r.oiy = o1 * r.iy;
     r.oiz = o2 * r.iz;
