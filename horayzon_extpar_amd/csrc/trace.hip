@@ -1,8 +1,7 @@
-// Horizon search kernel: one lane per cell, the whole elevation search of
-// that cell (all nhori * refine_factor azimuths) plus the clip, sector mean
-// and sky view factor fused behind it.  Replaces the TBB loop and the
-// per-ray Embree queries of the reference (mo_lradtopo_horayzon.cpp:686-761,
-// :490-555, :446-460).
+// Horizon search kernels: the elevation search of every cell (all nhori *
+// refine_factor azimuths) with the clip, sector mean and sky view factor
+// fused behind it.  Replaces the TBB loop and the per-ray Embree queries of
+// the reference (mo_lradtopo_horayzon.cpp:686-761, :490-555, :446-460).
 //
 // Numerics.  The walk state (ray direction, elevation angle, rotation axes)
 // is FP64 and is updated with the reference's Rodrigues rotations in the
@@ -14,13 +13,24 @@
 // (see lbvh.hip): its answer equals the OR of the triangle test over every
 // triangle, independent of the tree.
 //
-// Layout.  Lanes of a wavefront take cells that are neighbours in Morton
-// order, so their rays start close together and share BVH nodes in the
-// caches.  Workgroups are remapped so that each XCD (workgroups are dealt
-// round-robin over the 8 XCDs) walks one contiguous stretch of that order,
-// keeping each L2's working set spatially compact.  The traversal stack
-// keeps its first kLdsStack entries in LDS (entry-major, so lanes hit
-// distinct banks) and spills deeper entries to a private array.
+// k_trace_pt (default): persistent lanes.  Each lane runs one cell's search
+// as a state machine and keeps one occlusion query in flight; the wavefront
+// alternates between traversal steps (one node per live lane) and a refill
+// phase once fewer than `refill` lanes hold a live query, in which finished
+// lanes consume their answer, advance the search and issue the next query
+// (first against the cell's last occluding triangle, which answers about a
+// third of the hit queries without a traversal) or take the next cell.
+// Cells come from one work queue per XCD over its share of the Morton
+// order (workgroups are dealt round-robin over the 8 XCDs), so each L2 sees
+// a spatially compact stretch of cells.  The FP64 search state lives in a
+// per-lane global area while the lane traverses, which keeps the traversal
+// loop within 128 VGPRs (4 waves per SIMD).
+// k_trace: one lane per cell with the azimuths and queries in lock step
+// (kept as the simple reference schedule of the same arithmetic).
+//
+// The traversal stack keeps its first kLdsStack entries in LDS
+// (entry-major, so lanes hit distinct banks) and deeper entries in a private
+// array.
 
 #include "hz_internal.h"
 
