@@ -16,12 +16,13 @@ namespace hz {
 
 // Per-cell ray frame: unit sphere normal n and unit north direction (FP64,
 // reference mo_lradtopo_horayzon.cpp:693-712) and the FP32 ray origin
-// (:715-720).  64 bytes, one record per cell.
+// (:715-720).  64 bytes, one record per cell.  flags bit 0: the cell is a
+// corner of an ICON vertex with fewer than six cells (next to a mesh hole).
 struct alignas(16) CellFrame {
     double n[3];
     double north[3];
     float org[3];
-    int pad;
+    int pad;  // flags
 };
 static_assert(sizeof(CellFrame) == 64, "CellFrame layout");
 
@@ -179,7 +180,8 @@ int lbvh_build(LbvhWork& w, const float* vert, const int* tri, int ntri, int lea
                int morton_mode, int width, hipStream_t stream, DeviceBvh& out, bool want_depth,
                std::string& err);
 
-// Morton order of the cell ray origins (2D, x/y of the ENU frame).
+// Order of the cells: those next to a mesh hole first (CellFrame flag bit 0),
+// each group in Morton order of the ray origins (2D, x/y of the ENU frame).
 int order_cells(LbvhWork& w, const CellFrame* cells, int ncell, int* order, hipStream_t stream,
                 std::string& err);
 int order_reserve(LbvhWork& w, int ncell, std::string& err);

@@ -278,6 +278,22 @@ int prologue(const double* clon, const double* clat, const double* hsurf, const 
             f.pad = 0;
         }
     });
+    // --- cells at mesh holes ---------------------------------------------------
+    // A cell that is a corner of an ICON vertex with fewer than six cells (the
+    // domain border, pentagons) sits next to a hole of the mesh: its rays can
+    // leave through it and its walk then descends towards -85 deg, several
+    // times the queries of an interior cell.  Flag it (CellFrame::pad bit 0)
+    // so that the cell order can start these long searches first.
+    for (int v = 0; v < nvert; v++) {
+        int n = 0;
+        for (int j = 0; j < 6; j++)
+            if (cov[(size_t)nvert * j + v] != -2) n++;
+        if (n == 6) continue;
+        for (int j = 0; j < 6; j++) {
+            const int c = cov[(size_t)nvert * j + v];
+            if (c >= 0 && c < ncell) out.frame[c].pad |= 1;
+        }
+    }
     // --- cell -> adjacent triangles (slope / aspect) -------------------------
     out.adj_off.assign((size_t)ncell + 1, 0);
     for (long long t = 0; t < ntri; t++)
