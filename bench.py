@@ -144,11 +144,34 @@ def main():
     if learned_out:
         out["learned"] = learned_out
         out["learned_vs_plain_speedup"] = learned_out["speedup_vs_plain"]
+    out["end_to_end"] = end_to_end_leg(dom_in, opts)
     if not args.no_batch and mode == 0:
         out["batched_c4"] = batch_leg(args)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(dom_in, args.cpu_budget, args.cpu_threads)
     print(json.dumps(out), flush=True)
+
+
+def end_to_end_leg(dom_in, opts, reps=3):
+    """The drop-in call as a caller makes it: horizon_svf_comp on host
+    arrays (FP64 prologue, allocation and upload, LBVH, order, trace,
+    download, free), wall time of the C-ABI call; median of reps calls in a
+    process whose HIP runtime is already initialised."""
+    import numpy as np
+    from horayzon_extpar_amd.horizon_svf import horizon_svf_comp_py
+    walls, stats = [], []
+    for _ in range(reps):
+        st = {}
+        t0 = time.perf_counter()
+        horizon_svf_comp_py(*dom_in.args(), 24, 1, 40000.0, 0.2, 1, 2, stats=st, **opts)
+        walls.append(time.perf_counter() - t0)
+        stats.append(st)
+    k = int(np.argsort(walls)[len(walls) // 2])
+    st = stats[k]
+    return {"value": dom_in.ncell / walls[k], "unit": "cells/s", "wall_ms": walls[k] * 1e3,
+            "ms_prologue_host": st["t_prologue_ms"], "ms_alloc_upload": st["t_upload_ms"],
+            "ms_bvh": st["t_bvh_ms"], "ms_order": st["t_order_ms"], "ms_trace": st["t_trace_ms"],
+            "ms_download": st["t_download_ms"], "ms_total_library": st["t_total_ms"]}
 
 
 def learned_leg(dom, args, h_walk, s_walk, dom_in, plain_step_s):
