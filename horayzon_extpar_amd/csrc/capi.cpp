@@ -304,13 +304,6 @@ int hz_domain_create(const double* clon, const double* clat, const double* hsurf
     DCHK(hipMalloc(&d->d_skyview, (size_t)num_cell * sizeof(double)));
     DCHK(hipMalloc(&d->d_state, sizeof(TraceState)));
     DCHK(hipMalloc(&d->d_spill, trace_spill_doubles(num_cell) * sizeof(double)));
-    DCHK(hipMalloc(&d->d_adj_off, hs.adj_off.size() * sizeof(int)));
-    DCHK(hipMalloc(&d->d_adj, std::max<size_t>(hs.adj.size(), 1) * sizeof(int)));
-    DCHK(hipMemcpyAsync(d->d_adj_off, hs.adj_off.data(), hs.adj_off.size() * sizeof(int),
-                        hipMemcpyHostToDevice, d->stream));
-    if (!hs.adj.empty())
-        DCHK(hipMemcpyAsync(d->d_adj, hs.adj.data(), hs.adj.size() * sizeof(int),
-                            hipMemcpyHostToDevice, d->stream));
     DCHK(hipMalloc(&d->d_elev_cs, (kMaxLattice + 1) * (sizeof(double2) + sizeof(double))));
     {
         std::vector<double2> cs(kMaxLattice + 1);
@@ -422,6 +415,21 @@ int hz_domain_slope_aspect(hz_domain* d, double* slope, double* aspect) {
     g_err.clear();
     if (!d || !slope || !aspect) return fail(HZ_EINVAL, "NULL argument");
     RCHK(hipSetDevice(d->device));
+    if (!d->d_adj_off) {  // cell -> triangle adjacency, built on first use
+        std::vector<int> tri((size_t)d->ntri * 3), off, adj;
+        if (d->ntri)
+            RCHK(hipMemcpyAsync(tri.data(), d->d_tri, tri.size() * sizeof(int),
+                                hipMemcpyDeviceToHost, d->stream));
+        RCHK(hipStreamSynchronize(d->stream));
+        cell_adjacency(tri.data(), d->ntri, d->ncell, off, adj);
+        if (d->d_adj) RCHK(hipFree(d->d_adj));  // left by an earlier failed attempt
+        d->d_adj = nullptr;
+        RCHK(hipMalloc(&d->d_adj, std::max<size_t>(adj.size(), 1) * sizeof(int)));
+        if (!adj.empty())
+            RCHK(hipMemcpy(d->d_adj, adj.data(), adj.size() * sizeof(int), hipMemcpyHostToDevice));
+        RCHK(hipMalloc(&d->d_adj_off, off.size() * sizeof(int)));
+        RCHK(hipMemcpy(d->d_adj_off, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice));
+    }
     if (!d->d_slope) RCHK(hipMalloc(&d->d_slope, (size_t)d->ncell * 2 * sizeof(double)));
     slope_aspect(d->d_vert, d->d_tri, d->d_adj_off, d->d_adj, d->d_cells, d->ncell, d->d_slope,
                  d->d_slope + d->ncell, d->stream);

@@ -6,6 +6,8 @@
 
 #include <cstdint>
 #include <string>
+#include <memory>
+#include <utility>
 #include <vector>
 
 namespace hz {
@@ -124,12 +126,34 @@ void lattice_table(double hori_acc, double2* out, double* sin_h);
 // Host prologue (prologue.cpp)
 // ---------------------------------------------------------------------------
 
+// Allocator that leaves trivially constructible elements uninitialised on
+// resize: large host buffers are then first touched by the threads that fill
+// them, instead of being zeroed by one thread first.
+template <class T>
+struct UninitAlloc : std::allocator<T> {
+    template <class U>
+    struct rebind {
+        using other = UninitAlloc<U>;
+    };
+    UninitAlloc() = default;
+    template <class U>
+    UninitAlloc(const UninitAlloc<U>&) {}
+    template <class U>
+    void construct(U* p) noexcept {
+        ::new ((void*)p) U;
+    }
+    template <class U, class... A>
+    void construct(U* p, A&&... a) {
+        ::new ((void*)p) U(std::forward<A>(a)...);
+    }
+};
+template <class T>
+using HostVec = std::vector<T, UninitAlloc<T>>;
+
 struct HostScene {
-    std::vector<float> vert32;  // 3 floats per mesh vertex (ENU, FP32)
-    std::vector<int> tri;       // 3 vertex indices per triangle
-    std::vector<CellFrame> frame;
-    std::vector<int> adj_off;   // CSR: triangles with cell c as a corner are
-    std::vector<int> adj;       //      adj[adj_off[c] .. adj_off[c + 1])
+    HostVec<float> vert32;  // 3 floats per mesh vertex (ENU, FP32)
+    HostVec<int> tri;       // 3 vertex indices per triangle
+    HostVec<CellFrame> frame;
     long long num_mesh_vertices = 0;
     double lon0 = 0.0, lat0 = 0.0;
 };
@@ -140,6 +164,12 @@ int prologue(const double* clon, const double* clat, const double* hsurf, const 
 
 WalkParams make_walk_params(int num_hori, int refine_factor, int svf_type, double dist_search,
                             int ncell);
+
+// Cell -> adjacent triangles (CSR, triangles in increasing order): the
+// triangles with cell c (mesh vertex c < ncell) as a corner are
+// adj[off[c] .. off[c + 1]).  Needed by slope / aspect only.
+void cell_adjacency(const int* tri, long long ntri, int ncell, std::vector<int>& off,
+                    std::vector<int>& adj);
 
 // Slope / aspect [deg] per cell (terrain.hip)
 void slope_aspect(const float* vert, const int* tri, const int* off, const int* adj,

@@ -160,8 +160,8 @@ int prologue(const double* clon, const double* clat, const double* hsurf, const 
         return -1;
     }
     // mesh vertices as lon, lat, elevation (reference :234-238, :300-301)
-    std::vector<double> P((size_t)nmv * 3);
-    out.tri.assign((size_t)ntri * 3, 0);
+    HostVec<double> P((size_t)nmv * 3);  // every entry is written below
+    out.tri.resize((size_t)ntri * 3);
     parallel_for(ncell, [&](long long b, long long e) {
         for (long long i = b; i < e; i++) {
             P[3 * i] = clon[i];
@@ -294,25 +294,26 @@ int prologue(const double* clon, const double* clat, const double* hsurf, const 
             if (c >= 0 && c < ncell) out.frame[c].pad |= 1;
         }
     }
-    // --- cell -> adjacent triangles (slope / aspect) -------------------------
-    out.adj_off.assign((size_t)ncell + 1, 0);
-    for (long long t = 0; t < ntri; t++)
-        for (int k = 0; k < 3; k++) {
-            int v = out.tri[3 * t + k];
-            if (v < ncell) out.adj_off[v + 1]++;
-        }
-    for (int c = 0; c < ncell; c++) out.adj_off[c + 1] += out.adj_off[c];
-    out.adj.assign((size_t)out.adj_off[ncell], 0);
-    {
-        std::vector<int> fill(out.adj_off.begin(), out.adj_off.end() - 1);
-        for (long long t = 0; t < ntri; t++)
-            for (int k = 0; k < 3; k++) {
-                int v = out.tri[3 * t + k];
-                if (v < ncell) out.adj[fill[v]++] = (int)t;
-            }
-    }
     out.num_mesh_vertices = nmv;
     return 0;
+}
+
+void cell_adjacency(const int* tri, long long ntri, int ncell, std::vector<int>& off,
+                    std::vector<int>& adj) {
+    off.assign((size_t)ncell + 1, 0);
+    for (long long t = 0; t < ntri; t++)
+        for (int k = 0; k < 3; k++) {
+            const int v = tri[3 * t + k];
+            if (v < ncell) off[v + 1]++;
+        }
+    for (int c = 0; c < ncell; c++) off[c + 1] += off[c];
+    adj.assign((size_t)off[ncell], 0);
+    std::vector<int> fill(off.begin(), off.end() - 1);
+    for (long long t = 0; t < ntri; t++)
+        for (int k = 0; k < 3; k++) {
+            const int v = tri[3 * t + k];
+            if (v < ncell) adj[fill[v]++] = (int)t;
+        }
 }
 
 }  // namespace hz
