@@ -937,7 +937,11 @@ __global__ __launch_bounds__(BLOCK, 4) void k_trace_pt(const BvhNode* __restrict
             c1 = __builtin_amdgcn_s_memtime();
             cyc_refill += c1 - c0;
         }
-        const int thresh = exhausted ? 1 : refill;
+        // once the queues are empty, refill as soon as fewer lanes are live
+        // than hold a cell (scaled like the refill threshold), so that a lane
+        // that still has queries to make does not wait for the wave's slowest
+        int thresh = refill;
+        if (exhausted) thresh = max(1, (__popcll(__ballot(has_cell)) * refill + 63) >> 6);
         while (true) {
             if (COUNT) iters++;
             if (live) {
