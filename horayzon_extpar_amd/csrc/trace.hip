@@ -250,25 +250,31 @@ __device__ __forceinline__ int trav_step(const BvhNode* __restrict__ nodes,
     const int r0 = __float_as_int(d.x), r1 = __float_as_int(d.y);
     // leaf children: single triangles are inline (slot 0: d.zw e4 f4.xyz,
     // slot 1: f4.w g4 h4); larger leaves (MULTI only) live in the tris array.
-    // One test site for both slots keeps the register footprint small.
+    // Each slot has its own test site on the loaded registers: a shared site
+    // would need nine selects and a loop per step, and some lane of the wave
+    // tests a slot in nearly every step.
     const bool l0 = h0 && r0 < 0, l1 = h1 && r1 < 0;
     if (l0 || l1) {
-#pragma unroll 1
-        for (int s = 0; s < 2; s++) {
-            const bool act = s == 0 ? l0 : l1;
-            if (!act) continue;
-            const int ref = s == 0 ? r0 : r1;
-            if (MULTI && ref != ~(~ref & kLeafFirstMask)) {
-                if (leaf_hit<COUNT>(r, tris, ref, ntest, hit_leaf)) return 1;
-                continue;
+        if (l0) {
+            if (MULTI && r0 != ~(~r0 & kLeafFirstMask)) {
+                if (leaf_hit<COUNT>(r, tris, r0, ntest, hit_leaf)) return 1;
+            } else {
+                if (COUNT) ntest++;
+                if (tri_hit(r, d.z, d.w, e4.x, e4.y, e4.z, e4.w, f4.x, f4.y, f4.z)) {
+                    hit_leaf = ~r0;  // a single-triangle leaf: its leaf-order index
+                    return 1;
+                }
             }
-            if (COUNT) ntest++;
-            const bool z = s == 0;
-            if (tri_hit(r, z ? d.z : f4.w, z ? d.w : g4.x, z ? e4.x : g4.y, z ? e4.y : g4.z,
-                        z ? e4.z : g4.w, z ? e4.w : h4.x, z ? f4.x : h4.y, z ? f4.y : h4.z,
-                        z ? f4.z : h4.w)) {
-                hit_leaf = ~ref;  // a single-triangle leaf: its leaf-order index
-                return 1;
+        }
+        if (l1) {
+            if (MULTI && r1 != ~(~r1 & kLeafFirstMask)) {
+                if (leaf_hit<COUNT>(r, tris, r1, ntest, hit_leaf)) return 1;
+            } else {
+                if (COUNT) ntest++;
+                if (tri_hit(r, f4.w, g4.x, g4.y, g4.z, g4.w, h4.x, h4.y, h4.z, h4.w)) {
+                    hit_leaf = ~r1;
+                    return 1;
+                }
             }
         }
         h0 = h0 && r0 >= 0;
