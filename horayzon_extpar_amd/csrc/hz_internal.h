@@ -55,7 +55,8 @@ static_assert(sizeof(BvhNode) == 128, "BvhNode layout");
 //   q[6]           : lo x, hi x, lo y, hi y, lo z, hi z; byte k = child k
 //   ref[4]         : >= 0 interior node (bits 0..27), bit 30 set when that
 //                    node holds inline triangles; < 0 leaf, ~ref = code:
-//                    bit 30 set: inline triangle, slot = bit 0;
+//                    bit 30 set: inline triangle, slot = bit 0, leaf-order
+//                    index in bits 1..27;
 //                    else first | ((count - 1) << 27), triangles in leaf order
 //   tri[18]        : inline slots 0 (tri[0..8]) and 1 (tri[9..17])
 // The first 64 bytes hold everything but the inline triangles (and the first

@@ -540,7 +540,7 @@ __global__ __launch_bounds__(kBlock) void k_pack4(const int* __restrict__ sorted
             } else if (kind[k] == 1) {
                 if (inl < 2) {
                     put_tri(&nd.tri[9 * inl], vert, tri, sorted[kid[k]]);
-                    r = ~(kB4InlineBit | inl);
+                    r = ~(kB4InlineBit | (kid[k] << 1) | inl);
                     inl++;
                 } else {
                     r = ~kid[k];  // one triangle in the leaf-order array

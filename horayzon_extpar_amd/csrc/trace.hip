@@ -319,7 +319,7 @@ template <bool COUNT, int BLOCK>
 __device__ __forceinline__ int trav_step4(const Bvh4Node* __restrict__ nodes,
                                           const float4* __restrict__ tris, int* stk, int* ovf,
                                           const Ray& r, int& node, int& sp, unsigned& nvisit,
-                                          unsigned& ntest, int& err) {
+                                          unsigned& ntest, int& err, int& hit_leaf) {
     const float4* np = reinterpret_cast<const float4*>(nodes + (node & kLeafFirstMask));
     const float4 a = np[0], b = np[1], c = np[2], d = np[3];
     float4 e4 = make_float4(0.f, 0.f, 0.f, 0.f), f4 = e4, g4 = e4, h4 = e4;
@@ -371,45 +371,53 @@ __device__ __forceinline__ int trav_step4(const Bvh4Node* __restrict__ nodes,
         }
     }
     unsigned lh = hit & leafm;
-    // hit leaf children: inline triangles (slot 0: d.zw e4 f4.xyz, slot 1:
-    // f4.w g4 h4) or triangles of the leaf-order array; one test site
-#pragma unroll 1
-    while (lh) {
-        const int k = __builtin_ctz(lh);
-        lh &= lh - 1u;
-        const int code = ~(k == 0 ? r0 : (k == 1 ? r1 : (k == 2 ? r2 : r3)));
-        const bool inl = (code & kB4InlineBit) != 0;
-        const int first = code & kB4FirstMask;
-        const int cnt = inl ? 1 : (code >> kB4CountShift) + 1;
-#pragma unroll 1
-        for (int j = 0; j < cnt; j++) {
-            float v0, v1, v2, v3, v4, v5, v6, v7, v8;
-            if (inl) {
-                const bool z = (code & 1) == 0;
-                v0 = z ? d.z : f4.w;
-                v1 = z ? d.w : g4.x;
-                v2 = z ? e4.x : g4.y;
-                v3 = z ? e4.y : g4.z;
-                v4 = z ? e4.z : g4.w;
-                v5 = z ? e4.w : h4.x;
-                v6 = z ? f4.x : h4.y;
-                v7 = z ? f4.y : h4.z;
-                v8 = z ? f4.z : h4.w;
+    if (lh) {
+        // hit leaf children: the two inline triangles (slot 0: d.zw e4 f4.xyz,
+        // slot 1: f4.w g4 h4), each with a test site of its own, then leaves
+        // in the leaf-order array
+        int c0 = -1, c1 = -1;
+        unsigned rest = 0u;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            if (!((lh >> k) & 1u)) continue;
+            const int code = ~(k == 0 ? r0 : (k == 1 ? r1 : (k == 2 ? r2 : r3)));
+            if (code & kB4InlineBit) {
+                if (code & 1) c1 = code; else c0 = code;
             } else {
+                rest |= 1u << k;
+            }
+        }
+        if (c0 >= 0) {
+            if (COUNT) ntest++;
+            if (tri_hit(r, d.z, d.w, e4.x, e4.y, e4.z, e4.w, f4.x, f4.y, f4.z)) {
+                hit_leaf = (c0 >> 1) & kB4FirstMask;
+                return 1;
+            }
+        }
+        if (c1 >= 0) {
+            if (COUNT) ntest++;
+            if (tri_hit(r, f4.w, g4.x, g4.y, g4.z, g4.w, h4.x, h4.y, h4.z, h4.w)) {
+                hit_leaf = (c1 >> 1) & kB4FirstMask;
+                return 1;
+            }
+        }
+#pragma unroll 1
+        while (rest) {
+            const int k = __builtin_ctz(rest);
+            rest &= rest - 1u;
+            const int code = ~(k == 0 ? r0 : (k == 1 ? r1 : (k == 2 ? r2 : r3)));
+            const int first = code & kB4FirstMask;
+            const int cnt = (code >> kB4CountShift) + 1;
+#pragma unroll 1
+            for (int j = 0; j < cnt; j++) {
                 const float4* t = tris + 3 * (first + j);
                 const float4 t0 = t[0], t1 = t[1], t2 = t[2];
-                v0 = t0.x;
-                v1 = t0.y;
-                v2 = t0.z;
-                v3 = t0.w;
-                v4 = t1.x;
-                v5 = t1.y;
-                v6 = t1.z;
-                v7 = t1.w;
-                v8 = t2.x;
+                if (COUNT) ntest++;
+                if (tri_hit(r, t0.x, t0.y, t0.z, t0.w, t1.x, t1.y, t1.z, t1.w, t2.x)) {
+                    hit_leaf = first + j;
+                    return 1;
+                }
             }
-            if (COUNT) ntest++;
-            if (tri_hit(r, v0, v1, v2, v3, v4, v5, v6, v7, v8)) return 1;
         }
     }
     const unsigned ih = hit & ~leafm;
@@ -956,7 +964,7 @@ __global__ __launch_bounds__(BLOCK, 4) void k_trace_pt(const BvhNode* __restrict
                     if constexpr (TREE == 2)
                         s = trav_step4<COUNT, BLOCK>(reinterpret_cast<const Bvh4Node*>(lnodes),
                                                      ltris, stk, ovf, ray, node, sp, nvisit,
-                                                     ntest, err);
+                                                     ntest, err, hint);
                     else
                         s = trav_step<COUNT, BLOCK, TREE == 1>(lnodes, ltris, stk, ovf, ray, node,
                                                                sp, nvisit, ntest, err, hint);
