@@ -145,6 +145,7 @@ def main():
         out["learned"] = learned_out
         out["learned_vs_plain_speedup"] = learned_out["speedup_vs_plain"]
     out["end_to_end"] = end_to_end_leg(dom_in, opts)
+    out["end_to_end_device_prologue"] = end_to_end_leg(dom_in, dict(opts, prologue="device"))
     if not args.no_batch and mode == 0:
         out["batched_c4"] = batch_leg(args)
     if not args.no_cpu:
@@ -156,7 +157,9 @@ def end_to_end_leg(dom_in, opts, reps=3):
     """The drop-in call as a caller makes it: horizon_svf_comp on host
     arrays (FP64 prologue, allocation and upload, LBVH, order, trace,
     download, free), wall time of the C-ABI call; median of reps calls in a
-    process whose HIP runtime is already initialised."""
+    process whose HIP runtime is already initialised.  With
+    opts["prologue"] == "device" the prologue runs on the GPU (its time then
+    includes the upload of the raw ICON arrays)."""
     import numpy as np
     from horayzon_extpar_amd.horizon_svf import horizon_svf_comp_py
     walls, stats = [], []

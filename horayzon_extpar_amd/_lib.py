@@ -33,7 +33,8 @@ class HzOptions(ctypes.Structure):
                 ("cell_order", ctypes.c_int), ("verbose", ctypes.c_int),
                 ("trace_kernel", ctypes.c_int), ("refill", ctypes.c_int),
                 ("bvh_morton", ctypes.c_int), ("queue_chunk", ctypes.c_int),
-                ("bvh_width", ctypes.c_int), ("reserved", ctypes.c_int * 5)]
+                ("bvh_width", ctypes.c_int), ("prologue", ctypes.c_int),
+                ("reserved", ctypes.c_int * 4)]
 
 
 class HzStats(ctypes.Structure):
@@ -116,6 +117,11 @@ SIGNATURES = [
      [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint16), ctypes.c_int,
       ctypes.POINTER(ctypes.c_float)]),
     ("hz_domain_num_triangles", ctypes.c_longlong, [ctypes.c_void_p]),
+    ("hz_domain_scene", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float), c_int_p, c_double_p, c_double_p,
+      ctypes.POINTER(ctypes.c_float)]),
+    ("hz_sincos", ctypes.c_int,
+     [c_double_p, ctypes.c_longlong, c_double_p, c_double_p, ctypes.c_int]),
 ]
 
 
@@ -141,12 +147,22 @@ def last_error() -> str:
     return lib().hz_last_error().decode("utf-8", "replace")
 
 
+HZ_PROLOGUE = {"host": 0, "device": 1}
+
+
 def default_options(**kw) -> HzOptions:
+    """The library defaults with the given fields replaced (None keeps the
+    default); `prologue` also takes "host" / "device"."""
     o = HzOptions()
     lib().hz_default_options(ctypes.byref(o))
     for k, v in kw.items():
-        if v is not None:
-            setattr(o, k, v)
+        if v is None:
+            continue
+        if k == "prologue" and isinstance(v, str):
+            if v not in HZ_PROLOGUE:
+                raise ValueError(f"prologue must be one of {sorted(HZ_PROLOGUE)}")
+            v = HZ_PROLOGUE[v]
+        setattr(o, k, v)
     return o
 
 

@@ -189,6 +189,7 @@ void hz_default_options(hz_options* o) {
     o->refill = 40;
     o->queue_chunk = 0;
     o->bvh_width = 2;
+    o->prologue = HZ_PROLOGUE_HOST;
 }
 
 void hz_domain_destroy(hz_domain* d) {
@@ -261,7 +262,6 @@ int hz_domain_create(const double* clon, const double* clat, const double* hsurf
                      const double* vlon, const double* vlat, const int* cov, int num_cell,
                      int num_vertex, int grid_type, double ray_org_elev, const hz_options* opt,
                      hz_domain** out, hz_stats* stats) {
-    (void)opt;
     g_err.clear();
     if (!out) return fail(HZ_EINVAL, "out is NULL");
     *out = nullptr;
@@ -269,12 +269,17 @@ int hz_domain_create(const double* clon, const double* clat, const double* hsurf
         return fail(HZ_EINVAL, "NULL input array");
     int rc = check_grid_args(cov, num_cell, num_vertex, grid_type, ray_org_elev);
     if (rc) return rc;
+    const int where = opt ? opt->prologue : HZ_PROLOGUE_HOST;
+    if (where != HZ_PROLOGUE_HOST && where != HZ_PROLOGUE_DEVICE)
+        return fail(HZ_EINVAL, "prologue must be HZ_PROLOGUE_HOST or HZ_PROLOGUE_DEVICE");
     double t0 = now_ms();
-    HostScene hs;
     std::string err;
-    rc = prologue(clon, clat, hsurf, vlon, vlat, cov, num_cell, num_vertex, grid_type,
-                  ray_org_elev, hs, err);
-    if (rc) return fail(HZ_EINVAL, err);
+    HostScene hs;
+    if (where == HZ_PROLOGUE_HOST) {
+        rc = prologue(clon, clat, hsurf, vlon, vlat, cov, num_cell, num_vertex, grid_type,
+                      ray_org_elev, hs, err);
+        if (rc) return fail(HZ_EINVAL, err);
+    }
     double t1 = now_ms();
     hz_domain* d = new hz_domain();
     {
@@ -290,14 +295,50 @@ int hz_domain_create(const double* clon, const double* clat, const double* hsurf
     for (auto& e : d->ev) DCHK(hipEventCreate(&e));
     d->ncell = num_cell;
     d->grid_type = grid_type;
-    d->ntri = (long long)hs.tri.size() / 3;
-    d->nmeshvert = hs.num_mesh_vertices;
-    d->lon0 = hs.lon0;
-    d->lat0 = hs.lat0;
-    size_t nv = hs.vert32.size(), nt = hs.tri.size();
-    DCHK(hipMalloc(&d->d_vert, std::max<size_t>(nv, 3) * sizeof(float)));
-    DCHK(hipMalloc(&d->d_tri, std::max<size_t>(nt, 3) * sizeof(int)));
-    DCHK(hipMalloc(&d->d_cells, (size_t)num_cell * sizeof(CellFrame)));
+    if (where == HZ_PROLOGUE_DEVICE) {
+        DeviceScene ds;
+        rc = prologue_device(clon, clat, hsurf, vlon, vlat, cov, num_cell, num_vertex, grid_type,
+                             ray_org_elev, d->stream, ds, err);
+        if (rc) {
+            hz_domain_destroy(d);
+            return fail(rc == -1 ? HZ_EINVAL : (rc == -3 ? HZ_ENOMEM : HZ_EDEVICE), err);
+        }
+        d->d_vert = ds.vert;
+        d->d_tri = ds.tri;
+        d->d_cells = ds.cells;
+        d->ntri = ds.ntri;
+        d->nmeshvert = ds.nmv;
+        d->lon0 = ds.lon0;
+        d->lat0 = ds.lat0;
+        std::memcpy(d->bounds_xy, ds.bounds_xy, sizeof(d->bounds_xy));
+        t1 = now_ms();
+    } else {
+        d->ntri = (long long)hs.tri.size() / 3;
+        d->nmeshvert = hs.num_mesh_vertices;
+        d->lon0 = hs.lon0;
+        d->lat0 = hs.lat0;
+        size_t nv = hs.vert32.size(), nt = hs.tri.size();
+        DCHK(hipMalloc(&d->d_vert, std::max<size_t>(nv, 3) * sizeof(float)));
+        DCHK(hipMalloc(&d->d_tri, std::max<size_t>(nt, 3) * sizeof(int)));
+        DCHK(hipMalloc(&d->d_cells, (size_t)num_cell * sizeof(CellFrame)));
+        float b[4] = {3e38f, 3e38f, -3e38f, -3e38f};
+        for (size_t i = 0; i < nv; i += 3) {
+            b[0] = std::min(b[0], hs.vert32[i]);
+            b[1] = std::min(b[1], hs.vert32[i + 1]);
+            b[2] = std::max(b[2], hs.vert32[i]);
+            b[3] = std::max(b[3], hs.vert32[i + 1]);
+        }
+        if (nv == 0) b[0] = b[1] = b[2] = b[3] = 0.0f;
+        std::memcpy(d->bounds_xy, b, sizeof(b));
+        if (nv)
+            DCHK(hipMemcpyAsync(d->d_vert, hs.vert32.data(), nv * sizeof(float),
+                                hipMemcpyHostToDevice, d->stream));
+        if (nt)
+            DCHK(hipMemcpyAsync(d->d_tri, hs.tri.data(), nt * sizeof(int), hipMemcpyHostToDevice,
+                                d->stream));
+        DCHK(hipMemcpyAsync(d->d_cells, hs.frame.data(), (size_t)num_cell * sizeof(CellFrame),
+                            hipMemcpyHostToDevice, d->stream));
+    }
     DCHK(hipMalloc(&d->d_order, (size_t)num_cell * sizeof(int)));
     DCHK(hipMalloc(&d->d_nodes, std::max<long long>(1, d->ntri - 1) * sizeof(BvhNode)));
     DCHK(hipMalloc(&d->d_tris, std::max<long long>(1, d->ntri) * 3 * sizeof(float4)));
@@ -312,20 +353,7 @@ int hz_domain_create(const double* clon, const double* clat, const double* hsurf
         DCHK(hipMemcpy(d->d_elev_cs, cs.data(), cs.size() * sizeof(double2), hipMemcpyHostToDevice));
         DCHK(hipMemcpy(d->d_elev_cs + cs.size(), sh.data(), sh.size() * sizeof(double),
                        hipMemcpyHostToDevice));
-        float b[4] = {3e38f, 3e38f, -3e38f, -3e38f};
-        for (size_t i = 0; i < nv; i += 3) {
-            b[0] = std::min(b[0], hs.vert32[i]);
-            b[1] = std::min(b[1], hs.vert32[i + 1]);
-            b[2] = std::max(b[2], hs.vert32[i]);
-            b[3] = std::max(b[3], hs.vert32[i + 1]);
-        }
-        if (nv == 0) b[0] = b[1] = b[2] = b[3] = 0.0f;
-        std::memcpy(d->bounds_xy, b, sizeof(b));
     }
-    if (nv) DCHK(hipMemcpyAsync(d->d_vert, hs.vert32.data(), nv * sizeof(float), hipMemcpyHostToDevice, d->stream));
-    if (nt) DCHK(hipMemcpyAsync(d->d_tri, hs.tri.data(), nt * sizeof(int), hipMemcpyHostToDevice, d->stream));
-    DCHK(hipMemcpyAsync(d->d_cells, hs.frame.data(), (size_t)num_cell * sizeof(CellFrame),
-                        hipMemcpyHostToDevice, d->stream));
     if (lbvh_reserve(d->work, (int)std::max<long long>(std::max<long long>(d->ntri, num_cell), 1), err)) {
         hz_domain_destroy(d);
         return fail(HZ_ENOMEM, err);
@@ -342,6 +370,40 @@ int hz_domain_create(const double* clon, const double* clat, const double* hsurf
         stats->lat0 = d->lat0;
     }
     *out = d;
+    return HZ_OK;
+}
+
+int hz_domain_scene(const hz_domain* d, float* vert32, int* tri, double* frame_n,
+                    double* frame_north, float* org) {
+    g_err.clear();
+    if (!d) return fail(HZ_EINVAL, "domain is NULL");
+    hipError_t e = hipSuccess;
+    if (vert32 && d->nmeshvert)
+        e = hipMemcpy(vert32, d->d_vert, (size_t)d->nmeshvert * 3 * sizeof(float),
+                      hipMemcpyDeviceToHost);
+    if (e == hipSuccess && tri && d->ntri)
+        e = hipMemcpy(tri, d->d_tri, (size_t)d->ntri * 3 * sizeof(int), hipMemcpyDeviceToHost);
+    std::vector<CellFrame> f;
+    if (e == hipSuccess && (frame_n || frame_north || org)) {
+        f.resize(d->ncell);
+        e = hipMemcpy(f.data(), d->d_cells, (size_t)d->ncell * sizeof(CellFrame),
+                      hipMemcpyDeviceToHost);
+    }
+    if (e != hipSuccess) return fail(HZ_EDEVICE, std::string("hz_domain_scene: ") + hipGetErrorString(e));
+    for (size_t i = 0; i < f.size(); i++)
+        for (int a = 0; a < 3; a++) {
+            if (frame_n) frame_n[3 * i + a] = f[i].n[a];
+            if (frame_north) frame_north[3 * i + a] = f[i].north[a];
+            if (org) org[3 * i + a] = f[i].org[a];
+        }
+    return HZ_OK;
+}
+
+int hz_sincos(const double* x, long long n, double* s, double* c, int on_device) {
+    g_err.clear();
+    if (n < 0 || (n > 0 && (!x || !s || !c))) return fail(HZ_EINVAL, "hz_sincos: bad arguments");
+    std::string err;
+    if (sincos_cr_eval(x, n, s, c, on_device, err)) return fail(HZ_EDEVICE, err);
     return HZ_OK;
 }
 

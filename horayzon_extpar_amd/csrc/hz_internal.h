@@ -166,6 +166,44 @@ int prologue(const double* clon, const double* clat, const double* hsurf, const 
 WalkParams make_walk_params(int num_hori, int refine_factor, int svf_type, double dist_search,
                             int ncell);
 
+// ENU origin and transform constants (reference :178-201, :624-644): the
+// origin is the centroid of the cell points summed in cell order; so/co and
+// sa/ca are sin/cos of its longitude / latitude; (x0, y0, z0) the surface
+// point below it; centre / pole the earth centre and north pole in ENU.
+struct EnuFrame {
+    double lon0, lat0;
+    double so, co, sa, ca;
+    double x0, y0, z0;
+    double centre[3], pole[3];
+};
+// from the ECEF cell points P[3 * i .. 3 * i + 2], i < ncell
+void enu_frame_from_sum(const double* P, int ncell, EnuFrame& E);
+// from the cell lon / lat / elevation (host sin / cos, as the host prologue)
+void enu_frame_host(const double* clon, const double* clat, const double* hsurf, int ncell,
+                    EnuFrame& E);
+
+// Device prologue (prologue_dev.hip): the same mesh, FP32 scene and frames
+// as prologue(), built on the device from the raw ICON arrays.  Longitude /
+// latitude sin and cos are correctly rounded (crtrig.h); the ENU origin is
+// enu_frame_host's, computed on host threads while the device works.
+struct DeviceScene {
+    float* vert = nullptr;      // FP32 ENU mesh vertices, 3 per vertex
+    int* tri = nullptr;         // 3 mesh vertex indices per triangle
+    CellFrame* cells = nullptr; // per-cell frame, origin and flags
+    long long ntri = 0, nmv = 0;
+    double lon0 = 0.0, lat0 = 0.0;
+    float bounds_xy[4] = {0.f, 0.f, 0.f, 0.f};  // x/y extent of the vertices
+};
+// Arrays of `out` are allocated here (hipMalloc); the caller frees them.
+int prologue_device(const double* clon, const double* clat, const double* hsurf,
+                    const double* vlon, const double* vlat, const int* cells_of_vertex,
+                    int num_cell, int num_vertex, int grid_type, double ray_org_elev,
+                    hipStream_t st, DeviceScene& out, std::string& err);
+// sin / cos of n arguments with the device prologue's correctly rounded
+// routine, on the device (device != 0) or on the host
+int sincos_cr_eval(const double* x, long long n, double* s, double* c, int device,
+                   std::string& err);
+
 // Cell -> adjacent triangles (CSR, triangles in increasing order): the
 // triangles with cell c (mesh vertex c < ncell) as a corner are
 // adj[off[c] .. off[c + 1]).  Needed by slope / aspect only.

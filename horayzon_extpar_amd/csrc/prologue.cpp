@@ -124,6 +124,54 @@ WalkParams make_walk_params(int num_hori, int refine_factor, int svf_type, doubl
     return p;
 }
 
+void enu_frame_from_sum(const double* P, int ncell, EnuFrame& E) {
+    // centroid of the cell points, summed in cell order (reference :624-634)
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    for (int i = 0; i < ncell; i++) {
+        sx += P[3 * (size_t)i];
+        sy += P[3 * (size_t)i + 1];
+        sz += P[3 * (size_t)i + 2];
+    }
+    double rad = std::sqrt(sx * sx + sy * sy + sz * sz);
+    E.lon0 = std::atan2(sy, sx);
+    E.lat0 = std::asin(sz / rad);
+    // ECEF -> ENU about the surface point below the origin (reference :178-201)
+    sin_cos(E.lon0, &E.so, &E.co);
+    sin_cos(E.lat0, &E.sa, &E.ca);
+    const double R = kRadEarth;
+    E.x0 = R * E.ca * E.co;
+    E.y0 = R * E.ca * E.so;
+    E.z0 = R * E.sa;
+    auto enu = [&](double x, double y, double z, double* r) {
+        double dx = x - E.x0, dy = y - E.y0, dz = z - E.z0;
+        r[0] = -E.so * dx + E.co * dy;
+        r[1] = -E.sa * E.co * dx - E.sa * E.so * dy + E.ca * dz;
+        r[2] = E.ca * E.co * dx + E.ca * E.so * dy + E.sa * dz;
+    };
+    // earth centre and north pole in ENU (reference :643-644)
+    enu(0.0, 0.0, 0.0, E.centre);
+    enu(0.0, 0.0, R, E.pole);
+}
+
+void enu_frame_host(const double* clon, const double* clat, const double* hsurf, int ncell,
+                    EnuFrame& E) {
+    // spherical ECEF of the cell points (reference :158-169), then the origin
+    HostVec<double> P((size_t)ncell * 3);
+    const double R = kRadEarth;
+    parallel_for(ncell, [&](long long b, long long e) {
+        for (long long i = b; i < e; i++) {
+            double sl, cl, sp, cp;
+            sin_cos(clon[i], &sl, &cl);
+            sin_cos(clat[i], &sp, &cp);
+            double h = hsurf[i];
+            P[3 * i] = (R + h) * cp * cl;
+            P[3 * i + 1] = (R + h) * cp * sl;
+            P[3 * i + 2] = (R + h) * sp;
+        }
+    });
+    enu_frame_from_sum(P.data(), ncell, E);
+}
+
 int prologue(const double* clon, const double* clat, const double* hsurf, const double* vlon,
              const double* vlat, const int* cov, int ncell, int nvert, int grid_type,
              double ray_org_elev, HostScene& out, std::string& err) {
@@ -209,35 +257,21 @@ int prologue(const double* clon, const double* clat, const double* hsurf, const 
             P[3 * i + 2] = (R + h) * sp;
         }
     });
-    // --- ENU origin: centroid of the cell points, summed in cell order
-    //     (reference :624-634) ------------------------------------------------
-    double sx = 0.0, sy = 0.0, sz = 0.0;
-    for (int i = 0; i < ncell; i++) {
-        sx += P[3 * (size_t)i];
-        sy += P[3 * (size_t)i + 1];
-        sz += P[3 * (size_t)i + 2];
-    }
-    double rad = std::sqrt(sx * sx + sy * sy + sz * sz);
-    double lon0 = std::atan2(sy, sx);
-    double lat0 = std::asin(sz / rad);
-    out.lon0 = lon0;
-    out.lat0 = lat0;
-    // --- ECEF -> ENU (reference :178-201) ------------------------------------
-    double so, co, sa, ca;
-    sin_cos(lon0, &so, &co);
-    sin_cos(lat0, &sa, &ca);
-    const double x0 = R * ca * co;
-    const double y0 = R * ca * so;
-    const double z0 = R * sa;
+    // --- ENU origin and transform (reference :178-201, :624-644) -----------
+    EnuFrame E;
+    enu_frame_from_sum(P.data(), ncell, E);
+    out.lon0 = E.lon0;
+    out.lat0 = E.lat0;
+    const double so = E.so, co = E.co, sa = E.sa, ca = E.ca;
+    const double x0 = E.x0, y0 = E.y0, z0 = E.z0;
     auto enu = [&](double x, double y, double z, double* r) {
         double dx = x - x0, dy = y - y0, dz = z - z0;
         r[0] = -so * dx + co * dy;
         r[1] = -sa * co * dx - sa * so * dy + ca * dz;
         r[2] = ca * co * dx + ca * so * dy + sa * dz;
     };
-    double centre[3], pole[3];
-    enu(0.0, 0.0, 0.0, centre);
-    enu(0.0, 0.0, R, pole);
+    const double* centre = E.centre;
+    const double* pole = E.pole;
     out.vert32.resize((size_t)nmv * 3);
     parallel_for(nmv, [&](long long b, long long e) {
         for (long long i = b; i < e; i++) {

@@ -101,8 +101,8 @@ def horizon_svf_comp_py(clon, clat, hsurf, vlon, vlat, cells_of_vertex, num_hori
                         grid_type, dist_search, ray_org_elev, refine_factor, svf_type, *,
                         search_mode=None, leaf_size=None, block_size=None,
                         collect_counters=None, trace_kernel=None, refill=None,
-                        bvh_morton=None, bvh_width=None, return_slope_aspect=False,
-                        stats=None):
+                        bvh_morton=None, bvh_width=None, prologue=None,
+                        return_slope_aspect=False, stats=None):
     """Compute the terrain horizon [deg] and sky view factor [-].
 
     Same contract as the reference binding (horizon_svf.pyx:15-124):
@@ -112,6 +112,8 @@ def horizon_svf_comp_py(clon, clat, hsurf, vlon, vlat, cells_of_vertex, num_hori
     ``stats``: optional dict that receives the library statistics.
     ``return_slope_aspect=True`` appends the terrain slope and aspect [deg]
     per cell (an extension; the reference returns only the two arrays).
+    ``prologue="device"`` builds the mesh, ENU scene and frames on the GPU
+    (default "host": the reference's values bit for bit).
     """
     (num_hori, grid_type, dist_search, ray_org_elev, refine_factor,
      svf_type) = validate(clon, clat, hsurf, vlon, vlat, cells_of_vertex, num_hori,
@@ -124,10 +126,10 @@ def horizon_svf_comp_py(clon, clat, hsurf, vlon, vlat, cells_of_vertex, num_hori
     opt = default_options(search_mode=search_mode, leaf_size=leaf_size,
                           block_size=block_size, collect_counters=collect_counters,
                           trace_kernel=trace_kernel, refill=refill, bvh_morton=bvh_morton,
-                          bvh_width=bvh_width)
+                          bvh_width=bvh_width, prologue=prologue)
     if return_slope_aspect:
         with ResidentDomain(clon, clat, hsurf, vlon, vlat, cells_of_vertex, grid_type,
-                            ray_org_elev) as dom:
+                            ray_org_elev, prologue=opt.prologue) as dom:
             kw = {f: getattr(opt, f) for f in ("search_mode", "leaf_size", "block_size",
                                                "collect_counters", "trace_kernel", "refill",
                                                "bvh_morton", "bvh_width")}
@@ -160,7 +162,7 @@ class ResidentDomain:
     """
 
     def __init__(self, clon, clat, hsurf, vlon, vlat, cells_of_vertex, grid_type=1,
-                 ray_org_elev=0.2):
+                 ray_org_elev=0.2, prologue="host"):
         validate(clon, clat, hsurf, vlon, vlat, cells_of_vertex, 24, grid_type, 40000.0,
                  ray_org_elev, 1, 2)
         self._arrays = tuple(np.ascontiguousarray(x) for x in
@@ -169,11 +171,12 @@ class ResidentDomain:
         self.ncell = c[0].size
         self.create_stats = HzStats()
         h = ctypes.c_void_p()
+        opt = default_options(prologue=prologue)
         rc = _lib.lib().hz_domain_create(
             _ptr(c[0], ctypes.c_double), _ptr(c[1], ctypes.c_double),
             _ptr(c[2], ctypes.c_double), _ptr(c[3], ctypes.c_double),
             _ptr(c[4], ctypes.c_double), _ptr(c[5], ctypes.c_int), self.ncell,
-            c[3].size, int(grid_type), float(ray_org_elev), None, ctypes.byref(h),
+            c[3].size, int(grid_type), float(ray_org_elev), ctypes.byref(opt), ctypes.byref(h),
             ctypes.byref(self.create_stats))
         check(rc)
         self._h = h
@@ -195,6 +198,21 @@ class ResidentDomain:
         self._predictor = predictor
         check(_lib.lib().hz_domain_set_predictor(self._h, predictor.handle if predictor
                                                  else None))
+
+    def scene(self):
+        """The FP32 scene and frames this domain traces, read back from the
+        device: dict vert32 (nmv, 3), tri (ntri, 3), frame_n / frame_north
+        (ncell, 3) FP64, org (ncell, 3) FP32 (the layout of scene_export)."""
+        nt = int(self.create_stats.num_triangles)
+        nv = int(self.create_stats.num_mesh_vertices)
+        out = dict(vert32=np.empty((nv, 3), np.float32), tri=np.empty((nt, 3), np.int32),
+                   frame_n=np.empty((self.ncell, 3)), frame_north=np.empty((self.ncell, 3)),
+                   org=np.empty((self.ncell, 3), np.float32))
+        check(_lib.lib().hz_domain_scene(
+            self._h, _ptr(out["vert32"], ctypes.c_float), _ptr(out["tri"], ctypes.c_int),
+            _ptr(out["frame_n"], ctypes.c_double), _ptr(out["frame_north"], ctypes.c_double),
+            _ptr(out["org"], ctypes.c_float)))
+        return out
 
     def descriptors(self):
         """Per-cell descriptors, BF16 bit patterns [ncell, 192]."""

@@ -26,6 +26,9 @@ extern "C" {
 #define HZ_EKERNEL (-4) /* a kernel reported an internal limit (e.g. stack depth) */
 
 /* Search strategies of the elevation-angle search. */
+#define HZ_PROLOGUE_HOST 0
+#define HZ_PROLOGUE_DEVICE 1
+
 #define HZ_MODE_WALK 0    /* the reference walk: carried elevation, 0.5 deg steps, -85 deg floor */
 #define HZ_MODE_CLIMB 1   /* clip-aware per-azimuth climb from 0 deg (independent azimuths) */
 #define HZ_MODE_LEARNED 2 /* lattice search started at a predicted horizon (needs a predictor
@@ -49,7 +52,13 @@ typedef struct hz_options {
                              (default), 4 = four-wide nodes with 8-bit quantized child boxes
                              (half the node fetches, same results); the lane-per-cell kernel
                              always uses 2 */
-    int reserved[5];
+    int prologue;         /* where the FP64 mesh / ENU / frame prologue runs:
+                             HZ_PROLOGUE_HOST (default): host threads, glibc sin / cos, the
+                             reference's values bit for bit;
+                             HZ_PROLOGUE_DEVICE: on the GPU from the raw ICON arrays, with
+                             correctly rounded sin / cos (glibc's differ in the last bit on
+                             ~0.2% of arguments); the ENU origin is still the host's */
+    int reserved[4];
 } hz_options;
 
 typedef struct hz_stats {
@@ -240,6 +249,25 @@ void hz_batch_destroy(hz_batch* b);
 /* Number of cells / mesh triangles of a resident domain. */
 int hz_domain_num_cell(const hz_domain* dom);
 long long hz_domain_num_triangles(const hz_domain* dom);
+
+/*
+ * The scene a resident domain traces, read back from the device (either
+ * prologue): vert32[3 * num_mesh_vertices], tri[3 * num_triangles],
+ * frame_n / frame_north [3 * num_cell] (FP64), org [3 * num_cell] (FP32).
+ * NULL arrays are skipped.  Same layout as hz_scene_export, which computes
+ * it on the host; the two are compared to check the device prologue.  No
+ * reference counterpart (the reference keeps its scene inside Embree, :406-414).
+ */
+int hz_domain_scene(const hz_domain* dom, float* vert32, int* tri, double* frame_n,
+                    double* frame_north, float* org);
+
+/*
+ * The device prologue's correctly rounded sin and cos (the replacement of
+ * the glibc sincos calls of lonlat2ecef, mo_lradtopo_horayzon.cpp:158-169),
+ * for n arguments with |x| < 2^20: on the device (on_device = 1) or the same
+ * code on the host (0).  For checking the routine against exact values.
+ */
+int hz_sincos(const double* x, long long n, double* s, double* c, int on_device);
 
 #ifdef __cplusplus
 }
