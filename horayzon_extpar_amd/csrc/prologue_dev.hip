@@ -21,6 +21,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <thread>
@@ -232,13 +233,19 @@ __global__ void k_bounds_init(unsigned* b) {
     }
 }
 
-// x / y extent of the FP32 vertices (min x, min y, max x, max y)
-__global__ void k_bounds_xy(const float* __restrict__ vert, long long nmv, unsigned* b) {
-    const long long i = (long long)blockIdx.x * kBlockP + threadIdx.x;
+// x / y extent of the FP32 vertices (min x, min y, max x, max y): a
+// grid-stride pass, wavefront and workgroup reductions, one atomic per
+// workgroup and value
+__global__ __launch_bounds__(kBlockP) void k_bounds_xy(const float* __restrict__ vert,
+                                                       long long nmv, unsigned* b) {
     unsigned kx0 = 0xffffffffu, ky0 = 0xffffffffu, kx1 = 0u, ky1 = 0u;
-    if (i < nmv) {
-        kx0 = kx1 = fkey(vert[3 * i]);
-        ky0 = ky1 = fkey(vert[3 * i + 1]);
+    for (long long i = (long long)blockIdx.x * kBlockP + threadIdx.x; i < nmv;
+         i += (long long)gridDim.x * kBlockP) {
+        const unsigned kx = fkey(vert[3 * i]), ky = fkey(vert[3 * i + 1]);
+        kx0 = min(kx0, kx);
+        ky0 = min(ky0, ky);
+        kx1 = max(kx1, kx);
+        ky1 = max(ky1, ky);
     }
     for (int o = 32; o > 0; o >>= 1) {
         kx0 = min(kx0, (unsigned)__shfl_xor((int)kx0, o, 64));
@@ -246,7 +253,22 @@ __global__ void k_bounds_xy(const float* __restrict__ vert, long long nmv, unsig
         kx1 = max(kx1, (unsigned)__shfl_xor((int)kx1, o, 64));
         ky1 = max(ky1, (unsigned)__shfl_xor((int)ky1, o, 64));
     }
+    __shared__ unsigned part[kBlockP / 64][4];
+    const int w = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
+        part[w][0] = kx0;
+        part[w][1] = ky0;
+        part[w][2] = kx1;
+        part[w][3] = ky1;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < kBlockP / 64; k++) {
+            kx0 = min(kx0, part[k][0]);
+            ky0 = min(ky0, part[k][1]);
+            kx1 = max(kx1, part[k][2]);
+            ky1 = max(ky1, part[k][3]);
+        }
         atomicMin(&b[0], kx0);
         atomicMin(&b[1], ky0);
         atomicMax(&b[2], kx1);
@@ -379,7 +401,8 @@ int prologue_device(const double* clon, const double* clat, const double* hsurf,
         hipLaunchKernelGGL(k_hole_flags, dim3(gridp(nvert)), dim3(kBlockP), 0, st, d_cov, nvert,
                            ncell, out.cells);
     hipLaunchKernelGGL(k_bounds_init, dim3(1), dim3(64), 0, st, d_b);
-    hipLaunchKernelGGL(k_bounds_xy, dim3(gridp(nmv)), dim3(kBlockP), 0, st, out.vert, nmv, d_b);
+    hipLaunchKernelGGL(k_bounds_xy, dim3(std::min(gridp(nmv), 1024)), dim3(kBlockP), 0, st,
+                       out.vert, nmv, d_b);
     PCHK(hipGetLastError());
     PCHK(hipMemcpyAsync(bk, d_b, sizeof(bk), hipMemcpyDeviceToHost, st));
     PCHK(hipStreamSynchronize(st));

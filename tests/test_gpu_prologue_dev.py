@@ -8,7 +8,8 @@ host prologue and the oracle.
   the FP64 unit vectors of the frames agree to 1e-15 (they are derived from
   sin / cos values that glibc rounds differently on ~0.2% of arguments);
 * horizons computed from the device scene equal the oracle's (C1) and the
-  host-prologue run's (C2) bit for bit;
+  host-prologue run's (C2) bit for bit; slope / aspect and the learned
+  mode's descriptors agree with the host-prologue domain's;
 * an argument outside the reduction's exact range is rejected.
 """
 
@@ -113,6 +114,17 @@ def test_slope_aspect_device_prologue(c1):
     m = sh > 0.01
     da = np.abs((ah[m] - ad[m] + 180.0) % 360.0 - 180.0)
     assert m.sum() > 1000 and np.max(da) <= 1e-8
+
+
+def test_descriptors_device_prologue(c1):
+    # the learned mode's raster uses the scene's x / y extent: same scene and
+    # extent give the same descriptors
+    print("case device prologue descriptors", flush=True)
+    out = []
+    for where in ("host", "device"):
+        with ResidentDomain(*c1.args(), grid_type=1, ray_org_elev=0.2, prologue=where) as dom:
+            out.append(dom.descriptors())
+    assert np.array_equal(out[0], out[1])
 
 
 def test_out_of_range_longitude_rejected():
