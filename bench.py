@@ -149,7 +149,10 @@ def main():
     if not args.no_batch and mode == 0:
         out["batched_c4"] = batch_leg(args)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(dom_in, args.cpu_budget, args.cpu_threads)
+        out["cpu_baseline"], h_orc, s_orc = cpu_baseline(dom_in, args.cpu_budget,
+                                                        args.cpu_threads)
+        if mode == 0:
+            out["accuracy"] = accuracy_leg(dom_in, h, s, h_orc, s_orc, args.cpu_threads)
     print(json.dumps(out), flush=True)
 
 
@@ -318,17 +321,57 @@ def border_affected(dom_in, num_hori, dist_search, margin_m=3000.0):
     return (np.minimum(tx, ty) < dist_search * 1.02) | near
 
 
+def accuracy_leg(dom_in, h, s, h_orc, s_orc, threads, nsample=48):
+    """The accuracy gate of the run (walk mode): the device horizons against
+    the oracle's on every cell the CPU baseline traced (bit-for-bit parity
+    expected), and against the FP64 half-plane true horizon (brute force over
+    all triangles) on nsample seeded cells: on angles that are neither
+    grazing (within 1e-3 deg of a lattice step) nor able to meet the mesh
+    holes at the border, the walk must return the lattice value below the
+    true horizon, 0.25 + 0.5 floor(h / 0.5) deg, or 0."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+
+    done = ~np.isnan(s_orc)
+    dh = np.abs(h[:, done] - h_orc[:, done])
+    out = {"oracle_cells_compared": int(done.sum()),
+           "oracle_angles_differing": int((dh > 0).sum()),
+           "oracle_max_abs_dh_deg": float(dh.max()) if dh.size else None,
+           "oracle_max_abs_dsvf": float(np.abs(s[done] - s_orc[done]).max()) if done.any()
+           else None}
+    rng = np.random.default_rng(11)
+    cells = np.sort(rng.choice(dom_in.ncell, nsample, replace=False))
+    t0 = time.perf_counter()
+    th = np.degrees(orc.true_horizon(dom_in, cells, num_hori=24, grid_type=1,
+                                     nthreads=threads or min(os.cpu_count() or 1, 16)))
+    exp = np.where(th > 0, 0.25 + 0.5 * np.floor(th / 0.5), 0.0)
+    clear = (np.abs(th / 0.5 - np.round(th / 0.5)) * 0.5 > 1e-3)
+    clear &= ~border_affected(dom_in, 24, 40000.0)[:, cells].T
+    got = h[:, cells].T
+    agree = np.abs(got - exp) < 1e-9
+    err = np.abs(got - np.maximum(th, 0.0))[clear]
+    out["fp64_brute_force"] = {
+        "cells": int(nsample), "clear_angles": int(clear.sum()),
+        "clear_angles_agreeing": int(agree[clear].sum()),
+        "grazing_or_border_angles": int((~clear).sum()),
+        "max_abs_err_vs_true_deg": float(err.max()) if err.size else None,
+        "seconds": time.perf_counter() - t0}
+    return out
+
+
 def cpu_baseline(dom_in, budget_s, threads):
     """The oracle (C++ port of the reference walk, FP32 CPU BVH, std::thread
     pool with dynamic chunks like TBB's auto-partitioning): the whole C2
     domain on all host threads (time-budgeted), plus one thread on a block of
-    cells at the domain centre."""
+    cells at the domain centre.  Returns the JSON object and the oracle's
+    horizon / skyview of the whole-domain run (NaN where not reached)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as orc
 
     nt = threads or min(os.cpu_count() or 1, 16)
     n = dom_in.ncell
-    _, _, st = orc.horizon_svf(dom_in, grid_type=1, nthreads=nt, time_budget_s=budget_s)
+    h_orc, s_orc, st = orc.horizon_svf(dom_in, grid_type=1, nthreads=nt, time_budget_s=budget_s)
     cells = st.cells_done
     b = n // 2
     _, _, s1 = orc.horizon_svf(dom_in, grid_type=1, nthreads=1, cell_begin=b, cell_end=b + 20000,
@@ -339,7 +382,7 @@ def cpu_baseline(dom_in, budget_s, threads):
             "rays_per_s": st.num_rays / st.t_trace,
             "single_thread": {"value": s1.cells_done / s1.t_trace, "unit": "cells/s",
                               "cores": 1, "sample": f"{s1.cells_done} cells at the domain "
-                                                    f"centre, {s1.t_trace:.1f} s"}}
+                                                    f"centre, {s1.t_trace:.1f} s"}}, h_orc, s_orc
 
 
 if __name__ == "__main__":
