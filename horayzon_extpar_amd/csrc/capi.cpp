@@ -11,12 +11,15 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <limits>
+#include <thread>
+#include <vector>
 
 #include "../../include/horayzon.h"
 #include "hz_internal.h"
@@ -110,6 +113,8 @@ struct hz_domain {
     int* d_adj_off = nullptr;  // cell -> adjacent triangles (slope / aspect)
     int* d_adj = nullptr;
     double* d_slope = nullptr;
+    void* stage = nullptr;  // pinned staging slots of the download (hipHostMalloc)
+    hipEvent_t stage_ev[3] = {nullptr, nullptr, nullptr};
     float bounds_xy[4] = {0, 0, 0, 0};  // scene x/y extent (raster of the descriptor)
     // learned mode
     const hz_predictor* predictor = nullptr;
@@ -200,6 +205,9 @@ void hz_domain_destroy(hz_domain* d) {
                     d->d_tris,   d->d_horizon, d->d_skyview, d->d_state,
                     d->d_spill,  d->d_elev_cs, d->d_desc,    d->d_pred,
                     d->d_adj_off, d->d_adj,    d->d_slope};
+    if (d->stage) (void)hipHostFree(d->stage);
+    for (hipEvent_t e : d->stage_ev)
+        if (e) (void)hipEventDestroy(e);
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     lbvh_free(d->work);
@@ -640,6 +648,82 @@ int hz_domain_run(hz_domain* d, int num_hori, double dist_search, int refine_fac
     return HZ_OK;
 }
 
+namespace {
+
+// Device -> pageable host copy of a large output: 16 MB slots of pinned
+// memory are filled by asynchronous copies on the domain's stream while up
+// to 8 host threads copy the previous slots into the destination (which also
+// spreads its first-touch page faults).  Plain hipMemcpyAsync below 8 MB.
+constexpr size_t kStageChunk = (size_t)16 << 20;
+constexpr int kStageSlots = 3;
+
+hipError_t staged_d2h(hz_domain* d, void* dst, const void* src, size_t bytes) {
+    if (bytes < ((size_t)8 << 20)) {
+        hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, d->stream);
+        return e != hipSuccess ? e : hipStreamSynchronize(d->stream);
+    }
+    hipError_t e = hipSuccess;
+    if (!d->stage) {
+        e = hipHostMalloc(&d->stage, kStageChunk * kStageSlots);
+        if (e != hipSuccess) {
+            d->stage = nullptr;
+            return e;
+        }
+        for (auto& ev : d->stage_ev) {
+            e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+            if (e != hipSuccess) return e;
+        }
+    }
+    char* slot = static_cast<char*>(d->stage);
+    const char* s = static_cast<const char*>(src);
+    char* out = static_cast<char*>(dst);
+    const int nch = (int)((bytes + kStageChunk - 1) / kStageChunk);
+    auto len = [&](int k) { return std::min(kStageChunk, bytes - (size_t)k * kStageChunk); };
+    auto issue = [&](int k) {
+        hipError_t r = hipMemcpyAsync(slot + (size_t)(k % kStageSlots) * kStageChunk,
+                                      s + (size_t)k * kStageChunk, len(k), hipMemcpyDeviceToHost,
+                                      d->stream);
+        return r != hipSuccess ? r : hipEventRecord(d->stage_ev[k % kStageSlots], d->stream);
+    };
+    unsigned hw = std::thread::hardware_concurrency();
+    const int nt = (int)std::max(1u, std::min(hw ? hw : 1u, 8u));
+    std::atomic<int> ready{-1};   // highest chunk whose device copy has completed
+    std::atomic<bool> abort{false};
+    std::vector<std::atomic<int>> done(nch);
+    for (auto& x : done) x.store(0);
+    std::vector<std::thread> th;
+    for (int t = 0; t < nt; t++)
+        th.emplace_back([&, t]() {
+            for (int k = 0; k < nch; k++) {
+                while (ready.load(std::memory_order_acquire) < k) {
+                    if (abort.load()) return;
+                    std::this_thread::yield();
+                }
+                const size_t n = len(k), b = n / nt * t, e2 = t == nt - 1 ? n : n / nt * (t + 1);
+                std::memcpy(out + (size_t)k * kStageChunk + b,
+                            slot + (size_t)(k % kStageSlots) * kStageChunk + b, e2 - b);
+                done[k].fetch_add(1, std::memory_order_release);
+            }
+        });
+    for (int k = 0; k < std::min(nch, kStageSlots) && e == hipSuccess; k++) e = issue(k);
+    for (int k = 0; k < nch && e == hipSuccess; k++) {
+        e = hipEventSynchronize(d->stage_ev[k % kStageSlots]);
+        if (e != hipSuccess) break;
+        ready.store(k, std::memory_order_release);
+        if (k + kStageSlots < nch) {
+            // the slot is free once every thread has copied chunk k out of it
+            while (done[k].load(std::memory_order_acquire) < nt) std::this_thread::yield();
+            e = issue(k + kStageSlots);
+        }
+    }
+    if (e != hipSuccess) abort.store(true);
+    for (auto& x : th) x.join();
+    if (e == hipSuccess) e = hipStreamSynchronize(d->stream);
+    return e;
+}
+
+}  // namespace
+
 int hz_domain_fetch(hz_domain* d, double* horizon, double* skyview, hz_stats* stats) {
     g_err.clear();
     if (!d) return fail(HZ_EINVAL, "domain is NULL");
@@ -647,12 +731,10 @@ int hz_domain_fetch(hz_domain* d, double* horizon, double* skyview, hz_stats* st
     RCHK(hipSetDevice(d->device));
     double t0 = now_ms();
     if (horizon)
-        RCHK(hipMemcpyAsync(horizon, d->d_horizon, (size_t)d->last_nhori * d->ncell * sizeof(double),
-                            hipMemcpyDeviceToHost, d->stream));
+        RCHK(staged_d2h(d, horizon, d->d_horizon,
+                        (size_t)d->last_nhori * d->ncell * sizeof(double)));
     if (skyview)
-        RCHK(hipMemcpyAsync(skyview, d->d_skyview, (size_t)d->ncell * sizeof(double),
-                            hipMemcpyDeviceToHost, d->stream));
-    RCHK(hipStreamSynchronize(d->stream));
+        RCHK(staged_d2h(d, skyview, d->d_skyview, (size_t)d->ncell * sizeof(double)));
     if (stats) stats->t_download_ms = now_ms() - t0;
     return HZ_OK;
 }
