@@ -1,8 +1,8 @@
 """ICON grid ingestion and EXTPAR-style output: the steps on either side of
 the horizon path.
 
-Reading follows what the reference's driver does before calling the path
-(verification.py:82-104, development.py:186-201):
+Reading follows what the reference's verification script does before
+calling the path (verification.py:82-104, development.py:186-201):
 
 * the grid file holds ``clon, clat, vlon, vlat`` [rad] and the connectivity
   ``vertex_of_cell`` (3, ncell) and ``cells_of_vertex`` (6, nvert), 1-based
@@ -12,8 +12,9 @@ Reading follows what the reference's driver does before calling the path
   nominal resolution 5050 km / (n 2^k) (verification.py:91-93);
 * the EXTPAR file holds ``topography_c`` (cell elevation [m]).
 
-Writing follows what the driver hands back to EXTPAR (verification.py:
-202-203): ``HORIZON`` (nhori, ncell) and ``SKYVIEW`` (ncell) as float32.
+Writing follows what the verification script hands back to EXTPAR
+(verification.py:202-203): ``HORIZON`` (nhori, ncell) and ``SKYVIEW``
+(ncell) as float32.
 
 Files are NetCDF classic (CDF-1 / 64-bit offset), read and written with
 scipy.io; the same variables can also live in a ``.npz`` fixture.  NetCDF-4
@@ -127,7 +128,7 @@ def read_grid(path: str) -> IconGrid:
 def read_topography(path: str, ncell: int | None = None) -> np.ndarray:
     """Cell elevation ``topography_c`` [m] of an EXTPAR file, squeezed to 1-D
     float64 (verification.py:97-99); checked against the grid size as the
-    driver does (:105-106)."""
+    verification script does (:105-106)."""
     v, _ = _read_vars(path, ("topography_c",))
     h = np.ascontiguousarray(np.squeeze(v["topography_c"]), np.float64)
     if ncell is not None and h.size != ncell:
@@ -231,8 +232,8 @@ def horizon_svf_from_files(grid_file: str, topo_file: str, num_hori: int = 24,
                            svf_type: int = 2, out_file: str | None = None, **opts):
     """Read an ICON grid and its EXTPAR topography, run the path and (if
     out_file is given) write the float32 EXTPAR fields; returns the float64
-    (horizon, skyview).  Defaults are the reference driver's
-    (verification.py:151-155)."""
+    (horizon, skyview).  Defaults are the verification
+    script's (verification.py:151-155)."""
     g = read_grid(grid_file)
     hsurf = read_topography(topo_file, g.ncell)
     horizon, skyview = horizon_svf_comp_py(g.clon, g.clat, hsurf, g.vlon, g.vlat,

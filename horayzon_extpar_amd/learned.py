@@ -7,7 +7,7 @@ of maximum terrain elevation angle, 0.5 .. 40 km, BF16); its output is the
 horizon per azimuth in degrees.  ``search_mode=2`` starts each azimuth's
 lattice search at the predicted elevation: the search climbs while the query
 hits and descends while it misses, so a wrong prediction costs queries, not
-accuracy; the plain search stays the fallback and the judge.
+accuracy; the plain search stays the fallback and decides every result.
 
 Training targets are horizons the method itself produces (the clip-aware
 lattice search, ``search_mode=1``) on synthetic terrains from fresh seeds.
