@@ -6,7 +6,8 @@
 * the FP64 "true horizon" (half-plane sweep over every triangle) of random
   core cells, quantised to the 0.5 deg lattice, equals the device walk on
   every angle that is not within 1e-3 deg of a lattice elevation;
-* lattice invariant and run-to-run determinism on all 977k cells.
+* lattice invariant and run-to-run determinism on all 977k cells;
+* the device prologue gives the host prologue's FP32 scene and horizons.
 """
 
 import numpy as np
@@ -32,6 +33,25 @@ def c3_run():
           f"bvh {st['t_bvh_ms']:.2f} ms", flush=True)
     assert np.array_equal(h, h2) and np.array_equal(s, s2)
     return d, h, s
+
+
+def test_c3_device_prologue(c3_run):
+    # the device prologue on the seam / hole-rich mesh: same FP32 scene and
+    # ray origins as the host prologue, frames to 1e-15, same horizons
+    from horayzon_extpar_amd.horizon_svf import scene_export
+    print("case c3 device prologue", flush=True)
+    d, h, s = c3_run
+    host = scene_export(*d.args(), grid_type=1)
+    with ResidentDomain(*d.args(), grid_type=1, prologue="device") as rd:
+        dev = rd.scene()
+        rd.run(NHORI, 40000.0, 1, 2)
+        hd, sd = rd.fetch()
+    assert np.array_equal(dev["tri"], host["tri"])
+    assert np.array_equal(dev["vert32"].view(np.int32), host["vert32"].view(np.int32))
+    assert np.array_equal(dev["org"].view(np.int32), host["org"].view(np.int32))
+    assert np.max(np.abs(dev["frame_n"] - host["frame_n"])) <= 1e-15
+    assert np.max(np.abs(dev["frame_north"] - host["frame_north"])) <= 1e-15
+    assert np.array_equal(hd, h) and np.max(np.abs(sd - s)) <= 1e-12
 
 
 def test_c3_lattice(c3_run):
