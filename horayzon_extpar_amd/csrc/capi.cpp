@@ -11,14 +11,12 @@
 
 #include <hip/hip_runtime.h>
 
-#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <limits>
-#include <thread>
 #include <vector>
 
 #include "../../include/horayzon.h"
@@ -113,8 +111,7 @@ struct hz_domain {
     int* d_adj_off = nullptr;  // cell -> adjacent triangles (slope / aspect)
     int* d_adj = nullptr;
     double* d_slope = nullptr;
-    void* stage = nullptr;  // pinned staging slots of the download (hipHostMalloc)
-    hipEvent_t stage_ev[3] = {nullptr, nullptr, nullptr};
+    Stager stager;  // pinned slots of the large uploads and downloads
     float bounds_xy[4] = {0, 0, 0, 0};  // scene x/y extent (raster of the descriptor)
     // learned mode
     const hz_predictor* predictor = nullptr;
@@ -205,9 +202,7 @@ void hz_domain_destroy(hz_domain* d) {
                     d->d_tris,   d->d_horizon, d->d_skyview, d->d_state,
                     d->d_spill,  d->d_elev_cs, d->d_desc,    d->d_pred,
                     d->d_adj_off, d->d_adj,    d->d_slope};
-    if (d->stage) (void)hipHostFree(d->stage);
-    for (hipEvent_t e : d->stage_ev)
-        if (e) (void)hipEventDestroy(e);
+    d->stager.release();
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     lbvh_free(d->work);
@@ -306,7 +301,7 @@ int hz_domain_create(const double* clon, const double* clat, const double* hsurf
     if (where == HZ_PROLOGUE_DEVICE) {
         DeviceScene ds;
         rc = prologue_device(clon, clat, hsurf, vlon, vlat, cov, num_cell, num_vertex, grid_type,
-                             ray_org_elev, d->stream, ds, err);
+                             ray_org_elev, d->stream, d->stager, ds, err);
         if (rc) {
             hz_domain_destroy(d);
             return fail(rc == -1 ? HZ_EINVAL : (rc == -3 ? HZ_ENOMEM : HZ_EDEVICE), err);
@@ -329,23 +324,11 @@ int hz_domain_create(const double* clon, const double* clat, const double* hsurf
         DCHK(hipMalloc(&d->d_vert, std::max<size_t>(nv, 3) * sizeof(float)));
         DCHK(hipMalloc(&d->d_tri, std::max<size_t>(nt, 3) * sizeof(int)));
         DCHK(hipMalloc(&d->d_cells, (size_t)num_cell * sizeof(CellFrame)));
-        float b[4] = {3e38f, 3e38f, -3e38f, -3e38f};
-        for (size_t i = 0; i < nv; i += 3) {
-            b[0] = std::min(b[0], hs.vert32[i]);
-            b[1] = std::min(b[1], hs.vert32[i + 1]);
-            b[2] = std::max(b[2], hs.vert32[i]);
-            b[3] = std::max(b[3], hs.vert32[i + 1]);
-        }
-        if (nv == 0) b[0] = b[1] = b[2] = b[3] = 0.0f;
-        std::memcpy(d->bounds_xy, b, sizeof(b));
-        if (nv)
-            DCHK(hipMemcpyAsync(d->d_vert, hs.vert32.data(), nv * sizeof(float),
-                                hipMemcpyHostToDevice, d->stream));
-        if (nt)
-            DCHK(hipMemcpyAsync(d->d_tri, hs.tri.data(), nt * sizeof(int), hipMemcpyHostToDevice,
-                                d->stream));
-        DCHK(hipMemcpyAsync(d->d_cells, hs.frame.data(), (size_t)num_cell * sizeof(CellFrame),
-                            hipMemcpyHostToDevice, d->stream));
+        std::memcpy(d->bounds_xy, hs.bounds_xy, sizeof(d->bounds_xy));
+        if (nv) DCHK(d->stager.copy(d->d_vert, hs.vert32.data(), nv * sizeof(float), true, d->stream));
+        if (nt) DCHK(d->stager.copy(d->d_tri, hs.tri.data(), nt * sizeof(int), true, d->stream));
+        DCHK(d->stager.copy(d->d_cells, hs.frame.data(), (size_t)num_cell * sizeof(CellFrame), true,
+                            d->stream));
     }
     DCHK(hipMalloc(&d->d_order, (size_t)num_cell * sizeof(int)));
     DCHK(hipMalloc(&d->d_nodes, std::max<long long>(1, d->ntri - 1) * sizeof(BvhNode)));
@@ -648,82 +631,6 @@ int hz_domain_run(hz_domain* d, int num_hori, double dist_search, int refine_fac
     return HZ_OK;
 }
 
-namespace {
-
-// Device -> pageable host copy of a large output: 16 MB slots of pinned
-// memory are filled by asynchronous copies on the domain's stream while up
-// to 8 host threads copy the previous slots into the destination (which also
-// spreads its first-touch page faults).  Plain hipMemcpyAsync below 8 MB.
-constexpr size_t kStageChunk = (size_t)16 << 20;
-constexpr int kStageSlots = 3;
-
-hipError_t staged_d2h(hz_domain* d, void* dst, const void* src, size_t bytes) {
-    if (bytes < ((size_t)8 << 20)) {
-        hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, d->stream);
-        return e != hipSuccess ? e : hipStreamSynchronize(d->stream);
-    }
-    hipError_t e = hipSuccess;
-    if (!d->stage) {
-        e = hipHostMalloc(&d->stage, kStageChunk * kStageSlots);
-        if (e != hipSuccess) {
-            d->stage = nullptr;
-            return e;
-        }
-        for (auto& ev : d->stage_ev) {
-            e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
-            if (e != hipSuccess) return e;
-        }
-    }
-    char* slot = static_cast<char*>(d->stage);
-    const char* s = static_cast<const char*>(src);
-    char* out = static_cast<char*>(dst);
-    const int nch = (int)((bytes + kStageChunk - 1) / kStageChunk);
-    auto len = [&](int k) { return std::min(kStageChunk, bytes - (size_t)k * kStageChunk); };
-    auto issue = [&](int k) {
-        hipError_t r = hipMemcpyAsync(slot + (size_t)(k % kStageSlots) * kStageChunk,
-                                      s + (size_t)k * kStageChunk, len(k), hipMemcpyDeviceToHost,
-                                      d->stream);
-        return r != hipSuccess ? r : hipEventRecord(d->stage_ev[k % kStageSlots], d->stream);
-    };
-    unsigned hw = std::thread::hardware_concurrency();
-    const int nt = (int)std::max(1u, std::min(hw ? hw : 1u, 8u));
-    std::atomic<int> ready{-1};   // highest chunk whose device copy has completed
-    std::atomic<bool> abort{false};
-    std::vector<std::atomic<int>> done(nch);
-    for (auto& x : done) x.store(0);
-    std::vector<std::thread> th;
-    for (int t = 0; t < nt; t++)
-        th.emplace_back([&, t]() {
-            for (int k = 0; k < nch; k++) {
-                while (ready.load(std::memory_order_acquire) < k) {
-                    if (abort.load()) return;
-                    std::this_thread::yield();
-                }
-                const size_t n = len(k), b = n / nt * t, e2 = t == nt - 1 ? n : n / nt * (t + 1);
-                std::memcpy(out + (size_t)k * kStageChunk + b,
-                            slot + (size_t)(k % kStageSlots) * kStageChunk + b, e2 - b);
-                done[k].fetch_add(1, std::memory_order_release);
-            }
-        });
-    for (int k = 0; k < std::min(nch, kStageSlots) && e == hipSuccess; k++) e = issue(k);
-    for (int k = 0; k < nch && e == hipSuccess; k++) {
-        e = hipEventSynchronize(d->stage_ev[k % kStageSlots]);
-        if (e != hipSuccess) break;
-        ready.store(k, std::memory_order_release);
-        if (k + kStageSlots < nch) {
-            // the slot is free once every thread has copied chunk k out of it
-            while (done[k].load(std::memory_order_acquire) < nt) std::this_thread::yield();
-            e = issue(k + kStageSlots);
-        }
-    }
-    if (e != hipSuccess) abort.store(true);
-    for (auto& x : th) x.join();
-    if (e == hipSuccess) e = hipStreamSynchronize(d->stream);
-    return e;
-}
-
-}  // namespace
-
 int hz_domain_fetch(hz_domain* d, double* horizon, double* skyview, hz_stats* stats) {
     g_err.clear();
     if (!d) return fail(HZ_EINVAL, "domain is NULL");
@@ -731,10 +638,11 @@ int hz_domain_fetch(hz_domain* d, double* horizon, double* skyview, hz_stats* st
     RCHK(hipSetDevice(d->device));
     double t0 = now_ms();
     if (horizon)
-        RCHK(staged_d2h(d, horizon, d->d_horizon,
-                        (size_t)d->last_nhori * d->ncell * sizeof(double)));
+        RCHK(d->stager.copy(horizon, d->d_horizon,
+                            (size_t)d->last_nhori * d->ncell * sizeof(double), false, d->stream));
     if (skyview)
-        RCHK(staged_d2h(d, skyview, d->d_skyview, (size_t)d->ncell * sizeof(double)));
+        RCHK(d->stager.copy(skyview, d->d_skyview, (size_t)d->ncell * sizeof(double), false,
+                            d->stream));
     if (stats) stats->t_download_ms = now_ms() - t0;
     return HZ_OK;
 }

@@ -157,6 +157,7 @@ struct HostScene {
     HostVec<CellFrame> frame;
     long long num_mesh_vertices = 0;
     double lon0 = 0.0, lat0 = 0.0;
+    float bounds_xy[4] = {0.f, 0.f, 0.f, 0.f};  // x / y extent of vert32 (min x, min y, max x, max y)
 };
 
 int prologue(const double* clon, const double* clat, const double* hsurf, const double* vlon,
@@ -182,6 +183,18 @@ void enu_frame_from_sum(const double* P, int ncell, EnuFrame& E);
 void enu_frame_host(const double* clon, const double* clat, const double* hsurf, int ncell,
                     EnuFrame& E);
 
+// Large host <-> device copies through pinned slots (stage.cpp): copy()
+// moves `bytes` between pageable host memory and device memory on stream
+// st and returns once the copy is complete.  One Stager per stream user;
+// its pinned slots are allocated on first use and kept until release().
+struct Stager {
+    static constexpr int kSlots = 3;
+    void* buf = nullptr;
+    hipEvent_t ev[kSlots] = {nullptr, nullptr, nullptr};
+    hipError_t copy(void* dst, const void* src, size_t bytes, bool to_device, hipStream_t st);
+    void release();
+};
+
 // Device prologue (prologue_dev.hip): the same mesh, FP32 scene and frames
 // as prologue(), built on the device from the raw ICON arrays.  Longitude /
 // latitude sin and cos are correctly rounded (crtrig.h); the ENU origin is
@@ -198,7 +211,7 @@ struct DeviceScene {
 int prologue_device(const double* clon, const double* clat, const double* hsurf,
                     const double* vlon, const double* vlat, const int* cells_of_vertex,
                     int num_cell, int num_vertex, int grid_type, double ray_org_elev,
-                    hipStream_t st, DeviceScene& out, std::string& err);
+                    hipStream_t st, Stager& sg, DeviceScene& out, std::string& err);
 // sin / cos of n arguments with the device prologue's correctly rounded
 // routine, on the device (device != 0) or on the host
 int sincos_cr_eval(const double* x, long long n, double* s, double* c, int device,

@@ -14,6 +14,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
+#include <mutex>
 #include <thread>
 
 #include "hz_internal.h"
@@ -273,7 +275,10 @@ int prologue(const double* clon, const double* clat, const double* hsurf, const 
     const double* centre = E.centre;
     const double* pole = E.pole;
     out.vert32.resize((size_t)nmv * 3);
+    std::mutex bmu;
+    float bx[4] = {3e38f, 3e38f, -3e38f, -3e38f};
     parallel_for(nmv, [&](long long b, long long e) {
+        float lb[4] = {3e38f, 3e38f, -3e38f, -3e38f};
         for (long long i = b; i < e; i++) {
             double r[3];
             enu(P[3 * i], P[3 * i + 1], P[3 * i + 2], r);
@@ -281,11 +286,23 @@ int prologue(const double* clon, const double* clat, const double* hsurf, const 
             P[3 * i + 1] = r[1];
             P[3 * i + 2] = r[2];
             // FP32 scene vertex (reference :406-410)
-            out.vert32[3 * i] = (float)r[0];
-            out.vert32[3 * i + 1] = (float)r[1];
+            const float x = (float)r[0], y = (float)r[1];
+            out.vert32[3 * i] = x;
+            out.vert32[3 * i + 1] = y;
             out.vert32[3 * i + 2] = (float)r[2];
+            lb[0] = std::min(lb[0], x);
+            lb[1] = std::min(lb[1], y);
+            lb[2] = std::max(lb[2], x);
+            lb[3] = std::max(lb[3], y);
         }
+        std::lock_guard<std::mutex> g(bmu);
+        bx[0] = std::min(bx[0], lb[0]);
+        bx[1] = std::min(bx[1], lb[1]);
+        bx[2] = std::max(bx[2], lb[2]);
+        bx[3] = std::max(bx[3], lb[3]);
     });
+    if (nmv == 0) bx[0] = bx[1] = bx[2] = bx[3] = 0.0f;
+    std::memcpy(out.bounds_xy, bx, sizeof(bx));
     // --- per-cell frame (reference :693-722) ---------------------------------
     out.frame.resize(ncell);
     parallel_for(ncell, [&](long long b, long long e) {

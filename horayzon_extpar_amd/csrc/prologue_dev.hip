@@ -304,7 +304,7 @@ bool args_in_range(const double* a, long long n) {
 int prologue_device(const double* clon, const double* clat, const double* hsurf,
                     const double* vlon, const double* vlat, const int* cov, int ncell,
                     int nvert, int grid_type, double ray_org_elev, hipStream_t st,
-                    DeviceScene& out, std::string& err) {
+                    Stager& sg, DeviceScene& out, std::string& err) {
     if (ncell < 1 || nvert < 0) {
         err = "num_cell must be >= 1 and num_vertex >= 0";
         return -1;
@@ -347,17 +347,13 @@ int prologue_device(const double* clon, const double* clat, const double* hsurf,
     toff = nadd_v + nv1;
     aoff = toff + nv1;
     PCHK(hipMalloc(&d_b, 4 * sizeof(unsigned)));
-    PCHK(hipMemcpyAsync(d_clon, clon, (size_t)ncell * sizeof(double), hipMemcpyHostToDevice, st));
-    PCHK(hipMemcpyAsync(d_clat, clat, (size_t)ncell * sizeof(double), hipMemcpyHostToDevice, st));
-    PCHK(hipMemcpyAsync(d_hsurf, hsurf, (size_t)ncell * sizeof(double), hipMemcpyHostToDevice,
-                        st));
+    PCHK(sg.copy(d_clon, clon, (size_t)ncell * sizeof(double), true, st));
+    PCHK(sg.copy(d_clat, clat, (size_t)ncell * sizeof(double), true, st));
+    PCHK(sg.copy(d_hsurf, hsurf, (size_t)ncell * sizeof(double), true, st));
     if (nvert > 0) {
-        PCHK(hipMemcpyAsync(d_vlon, vlon, (size_t)nvert * sizeof(double), hipMemcpyHostToDevice,
-                            st));
-        PCHK(hipMemcpyAsync(d_vlat, vlat, (size_t)nvert * sizeof(double), hipMemcpyHostToDevice,
-                            st));
-        PCHK(hipMemcpyAsync(d_cov, cov, (size_t)6 * nvert * sizeof(int), hipMemcpyHostToDevice,
-                            st));
+        PCHK(sg.copy(d_vlon, vlon, (size_t)nvert * sizeof(double), true, st));
+        PCHK(sg.copy(d_vlat, vlat, (size_t)nvert * sizeof(double), true, st));
+        PCHK(sg.copy(d_cov, cov, (size_t)6 * nvert * sizeof(int), true, st));
     }
     // triangle and added-vertex counts per ICON vertex, exclusive scans
     hipLaunchKernelGGL(k_valence, dim3(gridp((long long)nv1)), dim3(kBlockP), 0, st, d_cov,
