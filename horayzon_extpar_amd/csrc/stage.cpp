@@ -35,16 +35,18 @@ void Stager::release() {
 hipError_t Stager::copy(void* dst, const void* src, size_t bytes, bool to_device,
                         hipStream_t st) {
     const hipMemcpyKind kind = to_device ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost;
-    if (bytes < kDirect) {
+    auto plain = [&]() {
         hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, st);
         return e != hipSuccess ? e : hipStreamSynchronize(st);
-    }
+    };
+    if (bytes < kDirect) return plain();
     hipError_t e = hipSuccess;
     if (!buf) {
-        e = hipHostMalloc(&buf, kChunk * kSlots);
-        if (e != hipSuccess) {
+        // pinned memory is a limited resource: without it, copy directly
+        if (hipHostMalloc(&buf, kChunk * kSlots) != hipSuccess) {
             buf = nullptr;
-            return e;
+            (void)hipGetLastError();
+            return plain();
         }
         for (auto& x : ev) {
             e = hipEventCreateWithFlags(&x, hipEventDisableTiming);
@@ -70,21 +72,27 @@ hipError_t Stager::copy(void* dst, const void* src, size_t bytes, bool to_device
     std::vector<std::atomic<int>> done(nch);
     for (auto& x : done) x.store(0);
     std::vector<std::thread> th;
-    for (int t = 0; t < nt; t++)
-        th.emplace_back([&, t]() {
-            for (int k = 0; k < nch; k++) {
-                while (go.load(std::memory_order_acquire) < k) {
-                    if (abort.load()) return;
-                    std::this_thread::yield();
-                }
-                const size_t n = len(k), b = n / nt * t, en = t == nt - 1 ? n : n / nt * (t + 1);
-                if (to_device)
-                    std::memcpy(sl(k) + b, s + (size_t)k * kChunk + b, en - b);
-                else
-                    std::memcpy(d + (size_t)k * kChunk + b, sl(k) + b, en - b);
-                done[k].fetch_add(1, std::memory_order_release);
+    auto work = [&](int t) {
+        for (int k = 0; k < nch; k++) {
+            while (go.load(std::memory_order_acquire) < k) {
+                if (abort.load()) return;
+                std::this_thread::yield();
             }
-        });
+            const size_t n = len(k), b = n / nt * t, en = t == nt - 1 ? n : n / nt * (t + 1);
+            if (to_device)
+                std::memcpy(sl(k) + b, s + (size_t)k * kChunk + b, en - b);
+            else
+                std::memcpy(d + (size_t)k * kChunk + b, sl(k) + b, en - b);
+            done[k].fetch_add(1, std::memory_order_release);
+        }
+    };
+    try {
+        for (int t = 0; t < nt; t++) th.emplace_back(work, t);
+    } catch (...) {  // no threads: nothing was copied yet, copy directly
+        abort.store(true);
+        for (auto& x : th) x.join();
+        return plain();
+    }
     auto wait_done = [&](int k) {
         while (done[k].load(std::memory_order_acquire) < nt) std::this_thread::yield();
     };
