@@ -88,6 +88,17 @@ def test_validation_messages(kw, exc, msg):
     assert msg in str(ei.value)
 
 
+def test_reference_module_name():
+    # the reference's import line, verification.py: from horizon_svf import ...
+    import horizon_svf
+    from horayzon_extpar_amd.horizon_svf import horizon_svf_comp_py as impl
+    assert horizon_svf.horizon_svf_comp_py is impl
+    d = synthetic.lattice_domain(10.0, 10.0, 2.0)
+    with pytest.raises(TypeError) as ei:
+        horizon_svf.horizon_svf_comp_py(*_args(d, ray_org_elev=0.05))
+    assert "Minimal allowed value for 'ray_org_elev' is 0.1 m" in str(ei.value)
+
+
 def test_validation_order_and_arrays():
     d = synthetic.lattice_domain(10.0, 10.0, 2.0)
     a = list(_args(d))
