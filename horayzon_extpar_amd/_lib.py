@@ -120,6 +120,9 @@ SIGNATURES = [
     ("hz_domain_scene", ctypes.c_int,
      [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float), c_int_p, c_double_p, c_double_p,
       ctypes.POINTER(ctypes.c_float)]),
+    ("hz_domain_bvh", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_longlong),
+      ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_longlong)]),
     ("hz_sincos", ctypes.c_int,
      [c_double_p, ctypes.c_longlong, c_double_p, c_double_p, ctypes.c_int]),
 ]

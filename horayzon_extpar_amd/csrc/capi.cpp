@@ -631,6 +631,50 @@ int hz_domain_run(hz_domain* d, int num_hori, double dist_search, int refine_fac
     return HZ_OK;
 }
 
+int hz_domain_bvh(hz_domain* d, const hz_options* opt, void* nodes, long long* num_nodes,
+                  float* tris, long long* num_tris) {
+    g_err.clear();
+    if (!d) return fail(HZ_EINVAL, "domain is NULL");
+    hz_options o;
+    if (opt)
+        o = *opt;
+    else
+        hz_default_options(&o);
+    const int leaf = o.leaf_size ? o.leaf_size : 1;
+    if (leaf < 1 || leaf > kMaxLeafSize) return fail(HZ_EINVAL, "leaf_size must be in [1, 8]");
+    const int width = o.bvh_width ? o.bvh_width : 2;
+    if (width != 2 && width != 4) return fail(HZ_EINVAL, "bvh_width must be 2 or 4");
+    if (o.bvh_morton < 0 || o.bvh_morton > 1) return fail(HZ_EINVAL, "bvh_morton must be 0 or 1");
+    RCHK(hipSetDevice(d->device));
+    std::string err;
+    d->bvh.nodes = d->d_nodes;
+    d->bvh.tris = d->d_tris;
+    int rc = lbvh_build(d->work, d->d_vert, d->d_tri, (int)d->ntri, leaf, o.bvh_morton, width,
+                        d->stream, d->bvh, false, err);
+    if (rc) return fail(rc == -3 ? HZ_ENOMEM : (rc == -1 ? HZ_EINVAL : HZ_EDEVICE), err);
+    long long count = 0;
+    if (d->ntri > 0) {
+        if (width == 4) {
+            unsigned c = 0;
+            RCHK(hipMemcpyAsync(&c, d->bvh.d_num_nodes, sizeof(c), hipMemcpyDeviceToHost, d->stream));
+            RCHK(hipStreamSynchronize(d->stream));
+            count = c;
+        } else {
+            count = std::max<long long>(1, d->ntri - 1);
+        }
+    }
+    if (num_nodes) *num_nodes = count;
+    if (num_tris) *num_tris = d->ntri;
+    if (nodes && count)
+        RCHK(hipMemcpyAsync(nodes, d->d_nodes, (size_t)count * sizeof(BvhNode),
+                            hipMemcpyDeviceToHost, d->stream));
+    if (tris && d->ntri)
+        RCHK(hipMemcpyAsync(tris, d->d_tris, (size_t)d->ntri * 3 * sizeof(float4),
+                            hipMemcpyDeviceToHost, d->stream));
+    RCHK(hipStreamSynchronize(d->stream));
+    return HZ_OK;
+}
+
 int hz_domain_fetch(hz_domain* d, double* horizon, double* skyview, hz_stats* stats) {
     g_err.clear();
     if (!d) return fail(HZ_EINVAL, "domain is NULL");

@@ -221,6 +221,22 @@ class ResidentDomain:
             ctypes.POINTER(ctypes.c_uint16))))
         return out
 
+    def bvh(self, **opts):
+        """Builds this domain's traversal tree with the tree options in opts
+        (leaf_size, bvh_width, bvh_morton) and returns (nodes, tris): the
+        128-byte node records as a uint8 array (num_nodes, 128) and the
+        leaf-order triangles as float32 (num_tris, 12)."""
+        opt = default_options(**opts)
+        nn, nt = ctypes.c_longlong(), ctypes.c_longlong()
+        L = _lib.lib()
+        check(L.hz_domain_bvh(self._h, ctypes.byref(opt), None, ctypes.byref(nn), None,
+                              ctypes.byref(nt)))
+        nodes = np.empty((nn.value, 128), np.uint8)
+        tris = np.empty((nt.value, 12), np.float32)
+        check(L.hz_domain_bvh(self._h, ctypes.byref(opt), nodes.ctypes.data_as(ctypes.c_void_p),
+                              ctypes.byref(nn), _ptr(tris, ctypes.c_float), ctypes.byref(nt)))
+        return nodes, tris
+
     def slope_aspect(self):
         """Terrain slope and aspect [deg] per cell (aspect: downhill azimuth,
         clockwise from north)."""

@@ -262,6 +262,19 @@ int hz_domain_scene(const hz_domain* dom, float* vert32, int* tri, double* frame
                     double* frame_north, float* org);
 
 /*
+ * Builds the traversal tree of a resident domain with the tree options of
+ * opt (leaf_size, bvh_width, bvh_morton; NULL: defaults) and reads it back:
+ * nodes[128 * num_nodes] bytes of BvhNode (width 2) or Bvh4Node (width 4)
+ * records, root first, and tris[12 * num_tris] floats, the triangles in
+ * leaf order (v0 xyz, v1 xyz, v2 xyz, original index as int bits, 2 pad).
+ * NULL arrays are skipped (call once for the counts).  For checking the
+ * tree's structure; the reference's counterpart is Embree's internal build
+ * (rtcCommitScene, mo_lradtopo_horayzon.cpp:422), which it cannot read back.
+ */
+int hz_domain_bvh(hz_domain* dom, const hz_options* opt, void* nodes, long long* num_nodes,
+                  float* tris, long long* num_tris);
+
+/*
  * The device prologue's correctly rounded sin and cos (the replacement of
  * the glibc sincos calls of lonlat2ecef, mo_lradtopo_horayzon.cpp:158-169),
  * for n arguments with |x| < 2^20: on the device (on_device = 1) or the same
