@@ -118,7 +118,6 @@ struct hz_domain {
     Raster raster;
     uint16_t* d_desc = nullptr;
     float* d_pred = nullptr;
-    MlpWork mlpw;
     bool have_pred = false;
     LbvhWork work;
     DeviceBvh bvh;
@@ -207,7 +206,6 @@ void hz_domain_destroy(hz_domain* d) {
         if (b) (void)hipFree(b);
     lbvh_free(d->work);
     d->raster.free_all();
-    d->mlpw.free_all();
     for (auto& e : d->ev)
         if (e) (void)hipEventDestroy(e);
     if (d->stream) (void)hipStreamDestroy(d->stream);
@@ -424,13 +422,7 @@ static int predict_cells(hz_domain* d, std::string& err) {
         return HZ_EDEVICE;
     descriptors(d->raster, d->d_cells, n, d->d_desc, d->stream);
     if (d->predictor) {
-        const int chunk = 1 << 16;
-        if (d->mlpw.reserve(std::min(n, chunk), err)) return HZ_ENOMEM;
-        for (int s = 0; s < n; s += chunk) {
-            int m = std::min(chunk, n - s);
-            mlp_infer(d->predictor->net, d->d_desc + (size_t)s * kIn, m, d->mlpw,
-                      d->d_pred + (size_t)s * kOutPad, d->stream);
-        }
+        mlp_infer(d->predictor->net, d->d_desc, n, d->d_pred, d->stream);
         d->have_pred = true;
     }
     if (hipGetLastError() != hipSuccess) {

@@ -1,5 +1,5 @@
 // Per-cell descriptor of the learned elevation bracket: for each of 24
-// azimuths and 8 log-spaced radial rings (0.5 .. 40 km), the largest
+// azimuths and 8 log-spaced radial rings (0.25 .. 40 km), the largest
 // elevation angle of the terrain seen from the cell's ray origin, sampled on
 // a max-height raster of the FP32 scene.  (No reference counterpart.)
 //
@@ -9,22 +9,31 @@
 //   k_descriptor : one lane per cell, 24 x 8 x kSamples raster samples; the
 //       elevation angle uses the cell's own up vector, so earth curvature and
 //       the tilt of distant cells' frames are taken into account.
+//
+// The sample distances and the azimuth directions are the same for every
+// cell; the host evaluates them in FP64 and rounds them to FP32 (DescTables),
+// so every other step is plain FP32 arithmetic that a CPU restatement
+// repeats exactly.
+
+#include <cmath>
 
 #include "hz_internal.h"
 #include "mlp.h"
 
 namespace hz {
 
-const float kRingRadius[kDescRings] = {500.f, 940.f, 1750.f, 3270.f, 6120.f, 11400.f, 21400.f, 40000.f};
-
 namespace {
 
 constexpr int kSamples = 3;  // samples per ring
+constexpr int kSamplesAz = kDescRings * kSamples;  // samples per azimuth
 
-// ring k outer radius: 500 m * 80^(k / 7), i.e. kRingRadius
-__device__ __forceinline__ float ring_radius(int k) {
-    return 500.0f * exp2f(6.321928094887363f * (float)k / 7.0f);
-}
+// sample s (0-based) of ring k at dist[k * kSamples + s]; azimuth j
+// (clockwise from north) has direction cos[j] north + sin[j] east
+struct DescTables {
+    float dist[kSamplesAz];
+    float sin[kDescAz];
+    float cos[kDescAz];
+};
 
 __device__ __forceinline__ unsigned f2ord(float f) {
     unsigned u = __float_as_uint(f);
@@ -83,8 +92,8 @@ __global__ void k_raster_tris(const float* __restrict__ vert, const int* __restr
 __global__ __launch_bounds__(128) void k_descriptor(const unsigned* __restrict__ z, int nx, int ny,
                                                     float x0, float y0, float inv_pix,
                                                     const CellFrame* __restrict__ cells, int ncell,
-                                                    uint16_t* __restrict__ desc) {
-    int c = blockIdx.x * 128 + threadIdx.x;
+                                                    DescTables tab, uint16_t* __restrict__ desc) {
+    const int c = blockIdx.x * 128 + threadIdx.x;
     if (c >= ncell) return;
     const CellFrame f = cells[c];
     const float ox = f.org[0], oy = f.org[1], oz = f.org[2];
@@ -93,29 +102,54 @@ __global__ __launch_bounds__(128) void k_descriptor(const unsigned* __restrict__
     // east = north x n
     const float Ex = Ny * nz3 - Nz * ny3, Ey = Nz * nx3 - Nx * nz3;
     for (int j = 0; j < kDescAz; j++) {
-        float sphi, cphi;
-        sincosf((float)j * (6.283185307179586f / kDescAz), &sphi, &cphi);
+        const float sphi = tab.sin[j], cphi = tab.cos[j];
         const float hx = cphi * Nx + sphi * Ex, hy = cphi * Ny + sphi * Ey;
-        float rlo = kRingInner;
+        // the azimuth's raster reads are issued together (samples off the
+        // raster read pixel 0 and are dropped): they are the kernel's latency
+        unsigned zu[kSamplesAz];
+#pragma unroll
+        for (int i = 0; i < kSamplesAz; i++) {
+            const float d = tab.dist[i];
+            const float qx = ox + d * hx, qy = oy + d * hy;
+            const int ix = (int)floorf((qx - x0) * inv_pix + 0.5f);
+            const int iy = (int)floorf((qy - y0) * inv_pix + 0.5f);
+            const bool in = ix >= 0 && iy >= 0 && ix < nx && iy < ny;
+            zu[i] = z[in ? (size_t)iy * nx + ix : 0];
+            if (!in) zu[i] = kEmpty;
+        }
+        uint16_t* dst = desc + (size_t)c * kIn + j * kDescRings;
+#pragma unroll
         for (int k = 0; k < kDescRings; k++) {
-            const float rhi = ring_radius(k);
-            const float lr = __log2f(rhi / rlo);
-            float best = -90.0f;
-            for (int s = 1; s <= kSamples; s++) {
-                const float d = rlo * exp2f(lr * (float)s / (float)kSamples);
-                const float qx = ox + d * hx, qy = oy + d * hy;
-                const int ix = (int)floorf((qx - x0) * inv_pix + 0.5f);
-                const int iy = (int)floorf((qy - y0) * inv_pix + 0.5f);
-                if (ix < 0 || iy < 0 || ix >= nx || iy >= ny) continue;
-                const unsigned zu = z[(size_t)iy * nx + ix];
-                if (zu == kEmpty) continue;
-                const float vx = qx - ox, vy = qy - oy, vz = ord2f(zu) - oz;
+            // the sample of largest elevation atan2(vn, hor), hor > 0, is the
+            // one of largest vn / hor: compared as vn^2 hor_b^2 against
+            // vn_b^2 hor^2 by sign, so atan2f runs once per bin
+            bool any = false;
+            float bn = 0.0f, bq = 1.0f;  // vn and hor^2 of the best sample
+#pragma unroll
+            for (int s = 0; s < kSamples; s++) {
+                const unsigned u = zu[k * kSamples + s];
+                if (u == kEmpty) continue;
+                const float d = tab.dist[k * kSamples + s];
+                const float vx = (ox + d * hx) - ox, vy = (oy + d * hy) - oy, vz = ord2f(u) - oz;
                 const float vn = vx * nx3 + vy * ny3 + vz * nz3;
-                const float hor = sqrtf(fmaxf(vx * vx + vy * vy + vz * vz - vn * vn, 1e-6f));
-                best = fmaxf(best, atan2f(vn, hor) * 57.29577951308232f);
+                const float q = fmaxf(vx * vx + vy * vy + vz * vz - vn * vn, 1e-6f);
+                bool better = !any;
+                if (any) {
+                    if ((vn >= 0.0f) != (bn >= 0.0f)) {
+                        better = vn >= 0.0f;
+                    } else {
+                        const float l = vn * vn * bq, rr = bn * bn * q;
+                        better = vn >= 0.0f ? l > rr : l < rr;
+                    }
+                }
+                if (better) {
+                    bn = vn;
+                    bq = q;
+                }
+                any = true;
             }
-            desc[(size_t)c * kIn + j * kDescRings + k] = f2bf(best * 0.1f);
-            rlo = rhi;
+            const float best = any ? atan2f(bn, sqrtf(bq)) * 57.29577951308232f : -90.0f;
+            dst[k] = f2bf(best * 0.1f);
         }
     }
 }
@@ -160,8 +194,22 @@ int raster_build(Raster& r, const float* vert, const int* tri, int ntri, const f
 
 void descriptors(const Raster& r, const CellFrame* cells, int ncell, uint16_t* desc,
                  hipStream_t st) {
+    // ring k spans (R(k-1), R(k)] with R(k) = 500 m * 80^(k/7) and R(-1) =
+    // kRingInner; its samples sit at R(k-1) * (R(k) / R(k-1))^(s / kSamples),
+    // s = 1 .. kSamples
+    DescTables tab;
+    auto ring = [](int k) { return k < 0 ? (double)kRingInner : 500.0 * std::pow(80.0, k / 7.0); };
+    for (int k = 0; k < kDescRings; k++)
+        for (int s = 1; s <= kSamples; s++)
+            tab.dist[k * kSamples + s - 1] =
+                (float)(ring(k - 1) * std::pow(ring(k) / ring(k - 1), (double)s / kSamples));
+    for (int j = 0; j < kDescAz; j++) {
+        const double phi = j * (2.0 * 3.141592653589793 / kDescAz);
+        tab.sin[j] = (float)std::sin(phi);
+        tab.cos[j] = (float)std::cos(phi);
+    }
     hipLaunchKernelGGL(k_descriptor, dim3((ncell + 127) / 128), dim3(128), 0, st, r.z, r.nx, r.ny,
-                       r.x0, r.y0, 1.0f / r.pix, cells, ncell, desc);
+                       r.x0, r.y0, 1.0f / r.pix, cells, ncell, tab, desc);
 }
 
 }  // namespace hz

@@ -300,18 +300,21 @@ int hz_trainer_predict(hz_trainer* t, const unsigned short* X, int n, float* out
     if (rc) return rc;
     float* y = nullptr;
     LCHK(hipMalloc(&y, (size_t)std::min(n, chunk) * kOutPad * sizeof(float)));
+    struct Free {
+        float* p;
+        ~Free() { (void)hipFree(p); }
+    } free_y{y};  // also on the error returns below
     std::vector<float> tmp((size_t)std::min(n, chunk) * kOutPad);
     for (int b = 0; b < n; b += chunk) {
         int m = std::min(chunk, n - b);
         LCHK(hipMemcpyAsync(t->Xb, X + (size_t)b * kIn, (size_t)m * kIn * 2, hipMemcpyHostToDevice,
                             t->st));
-        mlp_infer(t->net, t->Xb, m, t->w, y, t->st);
+        mlp_infer(t->net, t->Xb, m, y, t->st);
         LCHK(hipMemcpyAsync(tmp.data(), y, (size_t)m * kOutPad * 4, hipMemcpyDeviceToHost, t->st));
         LCHK(hipStreamSynchronize(t->st));
         for (int i = 0; i < m; i++)
             std::memcpy(out + (size_t)(b + i) * kOut, tmp.data() + (size_t)i * kOutPad, kOut * 4);
     }
-    (void)hipFree(y);
     return HZ_OK;
 }
 

@@ -69,7 +69,7 @@ void mlp_gemm(const uint16_t* A, int lda, const uint16_t* B, int ldb, int M, int
 void mlp_transpose(const uint16_t* in, int rows, int cols, int ldi, uint16_t* out, int ldo,
                    hipStream_t st);
 void mlp_forward(const Mlp& net, const uint16_t* X, int M, MlpWork& w, hipStream_t st);
-void mlp_infer(const Mlp& net, const uint16_t* X, int M, MlpWork& w, float* out, hipStream_t st);
+void mlp_infer(const Mlp& net, const uint16_t* X, int M, float* out, hipStream_t st);
 void mlp_backward(const Mlp& net, const uint16_t* X, const float* T, int M, MlpWork& w,
                   MlpGrads& g, hipStream_t st);
 void mlp_adam(Mlp& net, MlpAdam& opt, const MlpGrads& g, float lr, hipStream_t st);
@@ -89,9 +89,8 @@ struct Raster {
     void free_all();
 };
 
-// Ring radii of the descriptor [m] (log-spaced, 0.5 .. 40 km) and the inner
-// radius of the first ring.
-extern const float kRingRadius[kDescRings];
+// Inner radius of the descriptor's first ring [m]; ring k (0 .. 7) ends at
+// 500 m * 80^(k/7), the last at 40 km.
 constexpr float kRingInner = 250.0f;
 
 int raster_build(Raster& r, const float* vert, const int* tri, int ntri, const float* bounds_host,

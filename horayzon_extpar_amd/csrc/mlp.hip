@@ -14,7 +14,9 @@
 //   D: lane l holds D[row = 4 (l / 16) + i][col = l % 16], i = 0..3
 // The epilogue adds a bias, applies GELU and writes any of: the FP32 result,
 // the FP32 pre-activation and the BF16 activation.  The backward GEMMs use
-// the same kernel on transposed copies (transpose kernel below).
+// the same kernel on transposed copies (transpose kernel below).  Inference
+// runs all three layers in one kernel (k_mlp_fused) with the same MFMA
+// chains, so its predictions equal those of the three GEMMs.
 
 #include <cstring>
 
@@ -204,6 +206,109 @@ __global__ void k_gather(const uint16_t* __restrict__ X, const float* __restrict
 
 inline dim3 gemm_grid(int M, int N) { return dim3((N + 63) / 64, (M + 63) / 64); }
 
+// Fused inference.  One workgroup (4 wavefronts) carries 64 rows through all
+// three layers; the rows' activations stay in one LDS buffer (X, then H1,
+// then H2, each overwritten once every wavefront has read it) and the BF16
+// weights (240 KB, resident in L2) are read by every workgroup.  Wavefront
+// w computes columns 64 w .. 64 w + 63 of the hidden layers (4 x 4 MFMA
+// tiles over all 64 rows) and rows 16 w .. 16 w + 15 of the output layer.
+// The MFMA chain per tile walks K in the order of k_gemm_nt, so the results
+// equal the three-GEMM path bit for bit.
+constexpr int kFRows = 64;
+constexpr int kFLd = kH1 + 8;  // LDS row stride (elements): +16 bytes against bank conflicts
+static_assert(kH1 == kH2 && kIn <= kH1, "one LDS buffer holds X, H1 and H2");
+
+template <int K>
+__device__ __forceinline__ void fused_tile(const uint16_t* s, const uint16_t* __restrict__ W,
+                                           int n0, int r, int g, f32x4 (&acc)[4][4]) {
+#pragma unroll
+    for (int k0 = 0; k0 < K; k0 += 32) {
+        bf16x8 a[4], b[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) b[j] = load8(W + (size_t)(n0 + 16 * j + r) * K + k0 + 8 * g, true);
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            uint4 v = *reinterpret_cast<const uint4*>(s + (16 * i + r) * kFLd + k0 + 8 * g);
+            __builtin_memcpy(&a[i], &v, 16);
+        }
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+    }
+}
+
+__device__ __forceinline__ void fused_store(uint16_t* s, const float* __restrict__ bias, int n0,
+                                            int r, int g, const f32x4 (&acc)[4][4]) {
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const int col = n0 + 16 * j + r;
+        const float bb = bias[col];
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) s[(16 * i + 4 * g + q) * kFLd + col] = f2bf(gelu(acc[i][j][q] + bb));
+    }
+}
+
+__global__ __launch_bounds__(256) void k_mlp_fused(const uint16_t* __restrict__ X, int M,
+                                                   const uint16_t* __restrict__ w1,
+                                                   const float* __restrict__ b1,
+                                                   const uint16_t* __restrict__ w2,
+                                                   const float* __restrict__ b2,
+                                                   const uint16_t* __restrict__ w3,
+                                                   const float* __restrict__ b3,
+                                                   float* __restrict__ out) {
+    __shared__ __attribute__((aligned(16))) uint16_t s[kFRows * kFLd];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int r = lane & 15, g = lane >> 4;
+    const int row0 = blockIdx.x * kFRows;
+    // X rows (zero beyond M), 16 bytes per thread and step
+    for (int i = threadIdx.x; i < kFRows * (kIn / 8); i += 256) {
+        const int row = i / (kIn / 8), c8 = i % (kIn / 8);
+        uint4 v = {0u, 0u, 0u, 0u};
+        if (row0 + row < M) v = *reinterpret_cast<const uint4*>(X + (size_t)(row0 + row) * kIn + 8 * c8);
+        *reinterpret_cast<uint4*>(s + row * kFLd + 8 * c8) = v;
+    }
+    __syncthreads();
+    const int n0 = 64 * wave;
+    f32x4 acc[4][4];
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++) acc[i][j] = (f32x4){0.f, 0.f, 0.f, 0.f};
+    fused_tile<kIn>(s, w1, n0, r, g, acc);
+    __syncthreads();  // every wavefront has read X
+    fused_store(s, b1, n0, r, g, acc);
+    __syncthreads();
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++) acc[i][j] = (f32x4){0.f, 0.f, 0.f, 0.f};
+    fused_tile<kH1>(s, w2, n0, r, g, acc);
+    __syncthreads();  // every wavefront has read H1
+    fused_store(s, b2, n0, r, g, acc);
+    __syncthreads();
+    // output layer: 16 rows x 32 columns per wavefront
+    f32x4 o[2] = {(f32x4){0.f, 0.f, 0.f, 0.f}, (f32x4){0.f, 0.f, 0.f, 0.f}};
+#pragma unroll
+    for (int k0 = 0; k0 < kH2; k0 += 32) {
+        bf16x8 a, b[2];
+        uint4 v = *reinterpret_cast<const uint4*>(s + (16 * wave + r) * kFLd + k0 + 8 * g);
+        __builtin_memcpy(&a, &v, 16);
+#pragma unroll
+        for (int j = 0; j < 2; j++) b[j] = load8(w3 + (size_t)(16 * j + r) * kH2 + k0 + 8 * g, true);
+#pragma unroll
+        for (int j = 0; j < 2; j++) o[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[j], o[j], 0, 0, 0);
+    }
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+        const int col = 16 * j + r;
+        const float bb = b3[col];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int row = row0 + 16 * wave + 4 * g + q;
+            if (row < M) out[(size_t)row * kOutPad + col] = o[j][q] + bb;
+        }
+    }
+}
 }  // namespace
 
 void mlp_gemm(const uint16_t* A, int lda, const uint16_t* B, int ldb, int M, int N, int K,
@@ -330,12 +435,12 @@ void mlp_forward(const Mlp& net, const uint16_t* X, int M, MlpWork& w, hipStream
              0, st);
 }
 
-// Inference only (no pre-activations kept): predictions [M][kOutPad] in out.
-void mlp_infer(const Mlp& net, const uint16_t* X, int M, MlpWork& w, float* out, hipStream_t st) {
-    mlp_gemm(X, kIn, net.w1b, kIn, M, kH1, kIn, net.b1, 1, nullptr, kH1, nullptr, w.h1, kH1, st);
-    mlp_gemm(w.h1, kH1, net.w2b, kH1, M, kH2, kH1, net.b2, 1, nullptr, kH2, nullptr, w.h2, kH2, st);
-    mlp_gemm(w.h2, kH2, net.w3b, kH2, M, kOutPad, kH2, net.b3, 0, out, kOutPad, nullptr, nullptr, 0,
-             st);
+// Inference only (one fused launch, no intermediates in memory):
+// predictions [M][kOutPad] in out.
+void mlp_infer(const Mlp& net, const uint16_t* X, int M, float* out, hipStream_t st) {
+    if (M > 0)
+        hipLaunchKernelGGL(k_mlp_fused, dim3((unsigned)((M + kFRows - 1) / kFRows)), dim3(256), 0,
+                           st, X, M, net.w1b, net.b1, net.w2b, net.b2, net.w3b, net.b3, out);
 }
 
 // Backward pass of the mean squared error against T [M][kOut]; gradients

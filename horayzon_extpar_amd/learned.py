@@ -3,7 +3,7 @@ training-data pipeline (no counterpart in the reference).
 
 The predictor is a 192 -> 256 -> 256 -> 24 GELU network.  Its input is the
 per-cell descriptor computed on the device (24 azimuths x 8 log-spaced rings
-of maximum terrain elevation angle, 0.5 .. 40 km, BF16); its output is the
+of maximum terrain elevation angle, 0.25 .. 40 km, BF16); its output is the
 horizon per azimuth in degrees.  ``search_mode=2`` starts each azimuth's
 lattice search at the predicted elevation: the search climbs while the query
 hits and descends while it misses, so a wrong prediction costs queries, not
