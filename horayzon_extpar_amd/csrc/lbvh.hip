@@ -597,8 +597,11 @@ __global__ __launch_bounds__(kBlock) void k_depth(const int* __restrict__ parent
 
 // --- cell ordering ---------------------------------------------------------
 
+// Per workgroup, the min / max of the cells' ray origins in x and y, laid
+// out like k_tri_bounds' partials (z slots 0) for k_reduce_bounds.
 __global__ __launch_bounds__(kBlock) void k_cell_bounds(const CellFrame* __restrict__ cells,
-                                                       int n, unsigned* __restrict__ bounds) {
+                                                       int n, float* __restrict__ part) {
+    __shared__ float red[4][kBlock / 64];
     int i = blockIdx.x * kBlock + threadIdx.x;
     float mn[2] = {3.0e38f, 3.0e38f}, mx[2] = {-3.0e38f, -3.0e38f};
     if (i < n) {
@@ -609,11 +612,22 @@ __global__ __launch_bounds__(kBlock) void k_cell_bounds(const CellFrame* __restr
             mn[k] = fminf(mn[k], __shfl_xor(mn[k], off, 64));
             mx[k] = fmaxf(mx[k], __shfl_xor(mx[k], off, 64));
         }
-    if ((threadIdx.x & 63) == 0 && mn[0] <= mx[0]) {
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
         for (int k = 0; k < 2; k++) {
-            atomicMin(&bounds[k], f2ord(mn[k]));
-            atomicMax(&bounds[3 + k], f2ord(mx[k]));
+            red[k][w] = mn[k];
+            red[2 + k][w] = mx[k];
         }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const int k = threadIdx.x;  // 0, 1: min x, y; 3, 4: max x, y; 2, 5: z
+        float v = 0.0f;
+        if (k != 2 && k != 5) {
+            const int r = k < 3 ? k : k - 1;
+            v = red[r][0];
+            for (int q = 1; q < kBlock / 64; q++) v = k < 3 ? fminf(v, red[r][q]) : fmaxf(v, red[r][q]);
+        }
+        part[blockIdx.x * 8 + k] = v;
     }
 }
 
@@ -797,8 +811,8 @@ int order_reserve(LbvhWork& w, int ncell, std::string& err) { return lbvh_reserv
 int order_cells(LbvhWork& w, const CellFrame* cells, int n, int* order, hipStream_t st,
                 std::string& err) {
     if (lbvh_reserve(w, n, err)) return -3;
-    hipLaunchKernelGGL(k_init_bounds, dim3(1), dim3(64), 0, st, w.bounds);
-    hipLaunchKernelGGL(k_cell_bounds, dim3(grid(n)), dim3(kBlock), 0, st, cells, n, w.bounds);
+    hipLaunchKernelGGL(k_cell_bounds, dim3(grid(n)), dim3(kBlock), 0, st, cells, n, w.part);
+    hipLaunchKernelGGL(k_reduce_bounds, dim3(1), dim3(kBlock), 0, st, w.part, grid(n), w.bounds);
     hipLaunchKernelGGL(k_cell_morton, dim3(grid(n)), dim3(kBlock), 0, st, cells, n, w.bounds,
                        w.keys[0], w.vals[0]);
     size_t tmp = w.sort_tmp_bytes;

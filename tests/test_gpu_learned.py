@@ -2,7 +2,8 @@
 
 * The BF16 MFMA forward pass and the backward pass of the predictor match
   the FP64 oracle (oracle/mlp_oracle.py, itself pinned by finite
-  differences) evaluated with the same BF16 roundings.
+  differences) evaluated with the same BF16 roundings; inference gives every
+  row the same result whatever the batch size and offset.
 * The native trainer lowers the loss.
 * search_mode=2 (lattice search started at the predicted horizon) equals the
   CPU oracle's mode 2 bit for bit given the device's predictions; with a
@@ -60,6 +61,20 @@ def test_gradients_match_oracle():
         r = _rel(gg[name], gr[name])
         print(f"grad {name}: rel err {r:.2e}", flush=True)
         assert r < 3e-2, name
+
+
+def test_inference_rows_independent_of_batch():
+    # the fused inference kernel carries 64 rows per workgroup (zero rows
+    # past the end) and the host splits at 65,536 rows: any batch size and
+    # offset gives every row the same prediction, bit for bit
+    print("case mlp inference ragged batches", flush=True)
+    tr = learned.Trainer(seed=8)
+    rng = np.random.default_rng(9)
+    X = learned.bf16_bits((rng.standard_normal((70000, 192)) * 0.6).astype(np.float32))
+    full = tr.predict(X)
+    assert np.all(np.isfinite(full))
+    for a, b in [(0, 1), (0, 63), (0, 65), (0, 130), (5, 69), (65530, 65600), (69999, 70000)]:
+        assert np.array_equal(tr.predict(X[a:b]), full[a:b]), (a, b)
 
 
 def test_training_reduces_loss():
