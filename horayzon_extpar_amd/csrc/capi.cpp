@@ -107,6 +107,9 @@ struct hz_domain {
     int last_nhori = 0; // rows written by the last run
     TraceState* d_state = nullptr;
     double* d_spill = nullptr;
+    size_t spill_doubles = 0;  // capacity of d_spill
+    double* d_hitem = nullptr;  // per-(azimuth, cell) results of the lattice searches
+    size_t hitem_doubles = 0;
     double2* d_elev_cs = nullptr;
     int* d_adj_off = nullptr;  // cell -> adjacent triangles (slope / aspect)
     int* d_adj = nullptr;
@@ -200,7 +203,7 @@ void hz_domain_destroy(hz_domain* d) {
     void* bufs[] = {d->d_vert,   d->d_tri,     d->d_cells,   d->d_order, d->d_nodes,
                     d->d_tris,   d->d_horizon, d->d_skyview, d->d_state,
                     d->d_spill,  d->d_elev_cs, d->d_desc,    d->d_pred,
-                    d->d_adj_off, d->d_adj,    d->d_slope};
+                    d->d_adj_off, d->d_adj,    d->d_slope,   d->d_hitem};
     d->stager.release();
     for (void* b : bufs)
         if (b) (void)hipFree(b);
@@ -333,7 +336,8 @@ int hz_domain_create(const double* clon, const double* clat, const double* hsurf
     DCHK(hipMalloc(&d->d_tris, std::max<long long>(1, d->ntri) * 3 * sizeof(float4)));
     DCHK(hipMalloc(&d->d_skyview, (size_t)num_cell * sizeof(double)));
     DCHK(hipMalloc(&d->d_state, sizeof(TraceState)));
-    DCHK(hipMalloc(&d->d_spill, trace_spill_doubles(num_cell) * sizeof(double)));
+    d->spill_doubles = trace_spill_doubles(num_cell);
+    DCHK(hipMalloc(&d->d_spill, d->spill_doubles * sizeof(double)));
     DCHK(hipMalloc(&d->d_elev_cs, (kMaxLattice + 1) * (sizeof(double2) + sizeof(double))));
     {
         std::vector<double2> cs(kMaxLattice + 1);
@@ -570,9 +574,32 @@ int hz_domain_run(hz_domain* d, int num_hori, double dist_search, int refine_fac
         return fail(HZ_EINVAL, "trace_kernel must be 0 or 1");
     int refill = o.refill ? o.refill : 40;
     if (refill < 1 || refill > 64) return fail(HZ_EINVAL, "refill must be in [1, 64]");
+    // the lattice searches (climb, learned start) run one work item per cell
+    // and azimuth in the persistent kernel: a per-item result buffer, and the
+    // search spill sized for the items' lanes
+    double* hitem = nullptr;
+    const long long items = (long long)d->ncell * num_hori * refine_factor;
+    if (o.search_mode != HZ_MODE_WALK && o.trace_kernel == 0 && items < (1LL << 30)) {
+        if ((size_t)items > d->hitem_doubles) {
+            if (d->d_hitem) RCHK(hipFree(d->d_hitem));
+            d->d_hitem = nullptr;
+            d->hitem_doubles = 0;
+            RCHK(hipMalloc(&d->d_hitem, (size_t)items * sizeof(double)));
+            d->hitem_doubles = (size_t)items;
+        }
+        const size_t need = trace_spill_doubles((int)items);
+        if (need > d->spill_doubles) {
+            RCHK(hipFree(d->d_spill));
+            d->d_spill = nullptr;
+            d->spill_doubles = 0;
+            RCHK(hipMalloc(&d->d_spill, need * sizeof(double)));
+            d->spill_doubles = need;
+        }
+        hitem = d->d_hitem;
+    }
     rc = launch_trace(d->bvh, d->d_cells, order, p, o.search_mode, o.trace_kernel, block, refill,
-                      o.collect_counters != 0, d->d_horizon, d->d_skyview, d->d_state, d->d_spill, st,
-                      err);
+                      o.collect_counters != 0, d->d_horizon, d->d_skyview, d->d_state, d->d_spill,
+                      hitem, st, err);
     if (rc) return fail(HZ_EDEVICE, err);
     RCHK(hipEventRecord(d->ev[3], st));
     TraceCounters c;

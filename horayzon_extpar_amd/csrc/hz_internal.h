@@ -322,12 +322,15 @@ int launch_trace_batch(const BatchDomain* doms, int ndom, int total, bool multi,
 int batch_grid(int mode, int block, bool multi, int width, int total);
 
 // spill: per-lane FP64 search state of the persistent kernel, at least
-// trace_spill_doubles(...) doubles
+// trace_spill_doubles(work items) doubles.  hitem (modes 1 and 2, persistent
+// kernel; may be null): ncell * num_hori * refine_factor doubles; the
+// persistent kernel then takes (cell, azimuth) work items, and the spill must
+// hold trace_spill_doubles(ncell * num_hori * refine_factor) doubles.
 int launch_trace(const DeviceBvh& bvh, const CellFrame* cells, const int* order,
                  const WalkParams& p, int mode, int kernel, int block_size, int refill,
                  bool counters, double* horizon, double* skyview, TraceState* state,
-                 double* spill, hipStream_t stream, std::string& err);
-size_t trace_spill_doubles(int ncell);
+                 double* spill, double* hitem, hipStream_t stream, std::string& err);
+size_t trace_spill_doubles(int nwork);
 constexpr int S_FIELDS_HOST = 13;  // 8-byte fields of the per-lane search spill
 
 }  // namespace hz

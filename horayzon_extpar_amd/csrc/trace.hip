@@ -24,7 +24,13 @@
 // order (workgroups are dealt round-robin over the 8 XCDs), so each L2 sees
 // a spatially compact stretch of cells.  The FP64 search state lives in a
 // per-lane global area while the lane traverses, which keeps the traversal
-// loop within 128 VGPRs (4 waves per SIMD).
+// loop within 128 VGPRs (4 waves per SIMD).  The lattice searches (climb,
+// learned start) begin every azimuth afresh, so for them a work item is one
+// (cell, azimuth) pair: a chunk of 64 items is one azimuth of 64
+// neighbouring cells (nearly parallel rays from nearby origins), the launch
+// has 24x more items to share out, and k_items_finish forms the sector means
+// and SVF from the per-item results.  The walk carries its elevation from one
+// azimuth to the next and keeps whole cells as items.
 // k_trace: one lane per cell with the azimuths and queries in lock step
 // (kept as the simple reference schedule of the same arithmetic).
 //
@@ -604,7 +610,8 @@ __global__ __launch_bounds__(BLOCK) void k_trace(const BvhNode* __restrict__ nod
 // advance their search (next elevation step, next azimuth, or the end of the
 // cell with its sector means and SVF written out) and issue their next query;
 // lanes whose cell is finished take the next cell from the wave's chunk of
-// the Morton-ordered cell list.  Chunks of 64 cells come from one atomic
+// the Morton-ordered cell list (with ITEMS: the next (cell, azimuth) item).
+// Chunks of 64 cells come from one atomic
 // counter per XCD, each over its own contiguous eighth of that list; a wave
 // whose eighth is used up moves on to the next one.  This keeps SIMD lanes
 // busy although searches need 1 to ~200 queries per azimuth, and keeps each
@@ -612,6 +619,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace(const BvhNode* __restrict__ nod
 // ---------------------------------------------------------------------------
 
 constexpr int kChunk = 64;
+constexpr int kItemCells = 64;  // cells per block of work items (ITEMS)
 enum { P_AZSTART = 0, P_FIRST = 1, P_UP = 2, P_DOWN = 3 };
 
 __global__ void k_clear_state(TraceState* ts) {
@@ -635,7 +643,11 @@ static_assert(S_FIELDS == S_FIELDS_HOST, "spill layout");
 // BATCH: the cells of several domains (same search parameters) form one
 // index space; doms[d] covers global indices [doms[d].start, doms[d + 1].start)
 // and each lane follows its own cell's domain (scene, outputs).
-template <int MODE, bool COUNT, int BLOCK, int TREE, bool BATCH>
+// ITEMS (search modes 1 and 2, one domain): the work items are (cell,
+// azimuth) pairs instead of cells, since these searches start every azimuth
+// afresh; item results go to hitem and k_items_finish forms the sector means
+// and the SVF.
+template <int MODE, bool COUNT, int BLOCK, int TREE, bool BATCH, bool ITEMS = false>
 __global__ __launch_bounds__(BLOCK, 4) void k_trace_pt(const BvhNode* __restrict__ nodes,
                                                     const float4* __restrict__ tris, int num_tri,
                                                     const CellFrame* __restrict__ cells,
@@ -646,15 +658,17 @@ __global__ __launch_bounds__(BLOCK, 4) void k_trace_pt(const BvhNode* __restrict
                                                     TraceState* __restrict__ ts,
                                                     double* __restrict__ spill,
                                                     const BatchDomain* __restrict__ doms, int ndom,
-                                                    int total) {
+                                                    int total, double* __restrict__ hitem) {
+    static_assert(!ITEMS || (MODE != 0 && !BATCH), "items: lattice searches of one domain");
     __shared__ int stk_all[kLdsStack * BLOCK];
     // the wave's part of the stack: stk[k * BLOCK + lane] (the base is uniform)
     int* stk = stk_all + (__builtin_amdgcn_readfirstlane(threadIdx.x) & ~63);
     int ovf[kStackMax - kLdsStack];
     const int lane = threadIdx.x & 63;
     const unsigned long long lt_mask = (1ull << lane) - 1ull;
-    const int N = BATCH ? total : P.ncell;  // size of the work index space
     const int L = P.num_hori * P.refine_factor;
+    // size of the work index space (the launcher keeps ncell * L below 2^31)
+    const int N = BATCH ? total : (ITEMS ? P.ncell * L : P.ncell);
     const double a = P.hori_acc;
     int* queue = ts->queue;
     const size_t lanes = (size_t)gridDim.x * BLOCK;
@@ -712,7 +726,7 @@ __global__ __launch_bounds__(BLOCK, 4) void k_trace_pt(const BvhNode* __restrict
                     const int rank = __popcll(need & lt_mask);
                     if (want && rank < avail) {
                         const int g = qb + rank;
-                        int cell = 0, dom = 0;
+                        int cell = 0, dom = 0, item_j = 0, item_p = 0;
                         const CellFrame* lcells = cells;
                         if (BATCH) {
                             int lo = 0, hi = ndom - 1;  // last domain with start <= g
@@ -728,6 +742,16 @@ __global__ __launch_bounds__(BLOCK, 4) void k_trace_pt(const BvhNode* __restrict
                             ltris = bd.tris;
                             have = bd.num_tri > 0;
                             lcells = bd.cells;
+                        } else if (ITEMS) {
+                            // blocks of kItemCells cells x L azimuths, azimuth-major
+                            // inside a block: a chunk of 64 items is one azimuth of
+                            // 64 neighbouring cells
+                            const int blk = g / (kItemCells * L);
+                            const int rem = g - blk * (kItemCells * L);
+                            const int nb = min(kItemCells, P.ncell - blk * kItemCells);
+                            item_j = rem / nb;
+                            item_p = blk * kItemCells + (rem - item_j * nb);
+                            cell = order ? order[item_p] : item_p;
                         } else {
                             cell = order ? order[g] : g;
                         }
@@ -736,7 +760,12 @@ __global__ __launch_bounds__(BLOCK, 4) void k_trace_pt(const BvhNode* __restrict
                         const CellFrame* f = lcells + cell;
                         const D3 n = {f->n[0], f->n[1], f->n[2]};
                         const D3 north = {f->north[0], f->north[1], f->north[2]};
-                        const D3 dir = rodrigues(north, n, P.shift_sin, P.shift_cos);
+                        D3 dir = rodrigues(north, n, P.shift_sin, P.shift_cos);
+                        // the azimuth's direction by the same sequence of rotations
+                        // as the cell-by-cell search (:550-551)
+                        if (ITEMS)
+                            for (int t = 0; t < item_j; t++)
+                                dir = rodrigues(dir, n, -P.azim_sin, P.azim_cos);
                         SV(S_DIR) = dir.x;
                         SV(S_DIR + 1) = dir.y;
                         SV(S_DIR + 2) = dir.z;
@@ -744,8 +773,8 @@ __global__ __launch_bounds__(BLOCK, 4) void k_trace_pt(const BvhNode* __restrict
                         SV(S_SECTOR) = 0.0;
                         SV(S_SVF) = 0.0;
                         SI(S_I0) = make_int2(cell, P_AZSTART);
-                        SI(S_I1) = make_int2(0, 0);
-                        SI(S_I2) = make_int2(0, 0);
+                        SI(S_I1) = make_int2(ITEMS ? item_j : 0, 0);
+                        SI(S_I2) = make_int2(0, ITEMS ? item_p : 0);
                         if (BATCH) SI(S_I3) = make_int2(dom, 0);
                     }
                     qb += min(__popcll(need), avail);
@@ -882,6 +911,11 @@ __global__ __launch_bounds__(BLOCK, 4) void k_trace_pt(const BvhNode* __restrict
                     }
                     // azimuth finished: clip, sector mean, SVF (:736-754)
                     if (h < 0.0) h = 0.0;
+                    if (ITEMS) {  // item p = jsec of the spill: its result, then the next item
+                        hitem[(size_t)j * P.ncell + jsec] = h;
+                        has_cell = false;
+                        break;
+                    }
                     sector += h;
                     if (++ksec == P.refine_factor) {
                         lhorizon[(size_t)jsec * lncell + cell] =
@@ -1003,7 +1037,39 @@ __global__ __launch_bounds__(BLOCK, 4) void k_trace_pt(const BvhNode* __restrict
     }
 }
 
-template <int MODE, bool COUNT, int BLOCK, int TREE, bool BATCH>
+// Sector means and SVF of the item results of one domain, cell by cell in
+// the order of the cell-by-cell search (:736-754).
+__global__ void k_items_finish(const double* __restrict__ hitem, const int* __restrict__ order,
+                               WalkParams P, double* __restrict__ horizon,
+                               double* __restrict__ skyview) {
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= P.ncell) return;
+    const int cell = order ? order[p] : p;
+    const int L = P.num_hori * P.refine_factor;
+    double sector = 0.0, svf = 0.0;
+    int ksec = 0, jsec = 0;
+    for (int j = 0; j < L; j++) {
+        const double h = hitem[(size_t)j * P.ncell + p];
+        sector += h;
+        if (++ksec == P.refine_factor) {
+            horizon[(size_t)jsec * P.ncell + cell] =
+                ((sector / kPi) * 180.0) / (double)P.refine_factor;
+            sector = 0.0;
+            ksec = 0;
+            jsec++;
+        }
+        const double sn = sin_horizon(P, h);
+        if (P.svf_type == 0)
+            svf += (1.0 - sn);
+        else if (P.svf_type == 1)
+            svf += (1.0 - (sn * sn));
+        else
+            svf += (1.0 - (sn * sn * sn));
+    }
+    skyview[cell] = svf / (double)L;
+}
+
+template <int MODE, bool COUNT, int BLOCK, int TREE, bool BATCH, bool ITEMS = false>
 int pt_grid(int ncell) {
     static const int occ_grid = []() {
         int dev = 0, cus = 0, per = 0;
@@ -1011,7 +1077,8 @@ int pt_grid(int ncell) {
         if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
             return 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                &per, reinterpret_cast<const void*>(&k_trace_pt<MODE, COUNT, BLOCK, TREE, BATCH>),
+                &per,
+                reinterpret_cast<const void*>(&k_trace_pt<MODE, COUNT, BLOCK, TREE, BATCH, ITEMS>),
                 BLOCK, 0) != hipSuccess)
             return 0;
         return std::max(1, per) * std::max(1, cus);
@@ -1024,11 +1091,24 @@ int pt_grid(int ncell) {
 template <int MODE, bool COUNT, int BLOCK, int TREE>
 void launch_pt(const DeviceBvh& bvh, const CellFrame* cells, const int* order,
                const WalkParams& p, int refill, double* horizon, double* skyview,
-               TraceState* ts, double* spill, hipStream_t st) {
+               TraceState* ts, double* spill, double* hitem, hipStream_t st) {
+    if constexpr (MODE != 0) {
+        if (hitem) {
+            const int items = p.ncell * p.num_hori * p.refine_factor;
+            hipLaunchKernelGGL((k_trace_pt<MODE, COUNT, BLOCK, TREE, false, true>),
+                               dim3(pt_grid<MODE, COUNT, BLOCK, TREE, false, true>(items)),
+                               dim3(BLOCK), 0, st, bvh.nodes, bvh.tris, bvh.num_tri, cells, order,
+                               p, refill, kChunk, horizon, skyview, ts, spill, nullptr, 0, 0,
+                               hitem);
+            hipLaunchKernelGGL(k_items_finish, dim3((p.ncell + 255) / 256), dim3(256), 0, st,
+                               hitem, order, p, horizon, skyview);
+            return;
+        }
+    }
     hipLaunchKernelGGL((k_trace_pt<MODE, COUNT, BLOCK, TREE, false>),
                        dim3(pt_grid<MODE, COUNT, BLOCK, TREE, false>(p.ncell)), dim3(BLOCK), 0,
                        st, bvh.nodes, bvh.tris, bvh.num_tri, cells, order, p, refill, kChunk,
-                       horizon, skyview, ts, spill, nullptr, 0, 0);
+                       horizon, skyview, ts, spill, nullptr, 0, 0, nullptr);
 }
 
 template <int MODE, int BLOCK, int TREE>
@@ -1037,7 +1117,7 @@ void launch_pt_batch(const BatchDomain* doms, int ndom, int total, const WalkPar
     hipLaunchKernelGGL((k_trace_pt<MODE, false, BLOCK, TREE, true>),
                        dim3(pt_grid<MODE, false, BLOCK, TREE, true>(total)), dim3(BLOCK), 0, st,
                        nullptr, nullptr, 0, nullptr, nullptr, p, refill, chunk, nullptr, nullptr,
-                       ts, spill, doms, ndom, total);
+                       ts, spill, doms, ndom, total, nullptr);
 }
 
 inline int tree_kind(bool multi, int width) { return width == 4 ? 2 : (multi ? 1 : 0); }
@@ -1045,18 +1125,18 @@ inline int tree_kind(bool multi, int width) { return width == 4 ? 2 : (multi ? 1
 template <int MODE, bool COUNT, int BLOCK>
 void launch_one(const DeviceBvh& bvh, const CellFrame* cells, const int* order,
                 const WalkParams& p, int kernel, bool multi, int refill, double* horizon,
-                double* skyview, TraceState* ts, double* spill, hipStream_t st) {
+                double* skyview, TraceState* ts, double* spill, double* hitem, hipStream_t st) {
     if (kernel == 0) {
         const int tk = tree_kind(multi, bvh.width);
         if (tk == 2)
             launch_pt<MODE, COUNT, BLOCK, 2>(bvh, cells, order, p, refill, horizon, skyview, ts,
-                                             spill, st);
+                                             spill, hitem, st);
         else if (tk == 1)
             launch_pt<MODE, COUNT, BLOCK, 1>(bvh, cells, order, p, refill, horizon, skyview, ts,
-                                             spill, st);
+                                             spill, hitem, st);
         else
             launch_pt<MODE, COUNT, BLOCK, 0>(bvh, cells, order, p, refill, horizon, skyview, ts,
-                                             spill, st);
+                                             spill, hitem, st);
         return;
     }
     if constexpr (MODE == 2) {
@@ -1072,16 +1152,16 @@ void launch_one(const DeviceBvh& bvh, const CellFrame* cells, const int* order,
 template <int MODE, bool COUNT>
 void launch_block(int block, const DeviceBvh& bvh, const CellFrame* cells, const int* order,
                   const WalkParams& p, int kernel, bool multi, int refill, double* horizon,
-                  double* skyview, TraceState* ts, double* spill, hipStream_t st) {
+                  double* skyview, TraceState* ts, double* spill, double* hitem, hipStream_t st) {
     if (block == 64)
         launch_one<MODE, COUNT, 64>(bvh, cells, order, p, kernel, multi, refill, horizon, skyview,
-                                    ts, spill, st);
+                                    ts, spill, hitem, st);
     else if (block == 256)
         launch_one<MODE, COUNT, 256>(bvh, cells, order, p, kernel, multi, refill, horizon,
-                                     skyview, ts, spill, st);
+                                     skyview, ts, spill, hitem, st);
     else
         launch_one<MODE, COUNT, 128>(bvh, cells, order, p, kernel, multi, refill, horizon,
-                                     skyview, ts, spill, st);
+                                     skyview, ts, spill, hitem, st);
 }
 
 template <int MODE, int BLOCK>
@@ -1159,8 +1239,8 @@ int launch_trace_batch(const BatchDomain* doms, int ndom, int total, bool multi,
 
 int launch_trace(const DeviceBvh& bvh, const CellFrame* cells, const int* order,
                  const WalkParams& p, int mode, int kernel, int block, int refill, bool counters,
-                 double* horizon, double* skyview, TraceState* ts, double* spill, hipStream_t st,
-                 std::string& err) {
+                 double* horizon, double* skyview, TraceState* ts, double* spill, double* hitem,
+                 hipStream_t st, std::string& err) {
     const bool multi = bvh.leaf_size > 1;
     if (refill < 1 || refill > 64) refill = 40;
     hipLaunchKernelGGL(k_clear_state, dim3(1), dim3(256), 0, st, ts);
@@ -1176,10 +1256,10 @@ int launch_trace(const DeviceBvh& bvh, const CellFrame* cells, const int* order,
     do {                                                                                      \
         if (counters)                                                                         \
             launch_block<M, true>(block, bvh, cells, order, p, kernel, multi, refill, horizon,   \
-                                  skyview, ts, spill, st);                                    \
+                                  skyview, ts, spill, hitem, st);                             \
         else                                                                                  \
             launch_block<M, false>(block, bvh, cells, order, p, kernel, multi, refill, horizon,  \
-                                   skyview, ts, spill, st);                                   \
+                                   skyview, ts, spill, hitem, st);                            \
     } while (0)
     if (mode == 0)
         HZ_LAUNCH(0);

@@ -25,7 +25,7 @@ def _report(name, h_gpu, h_orc):
 
 
 @pytest.mark.parametrize("grid_type", [1, 0])
-@pytest.mark.parametrize("svf_type", [2, 0])
+@pytest.mark.parametrize("svf_type", [2, 1, 0])
 def test_c1_walk_bit_exact(c1, grid_type, svf_type):
     print(f"case c1 walk gt={grid_type} svf={svf_type}", flush=True)
     st = {}
@@ -47,13 +47,21 @@ def test_c1_refine_factor_10(c1):
     assert np.max(np.abs(s - so)) <= SVF_TOL
 
 
-def test_c1_climb_mode(c1):
-    print("case c1 climb", flush=True)
-    h, s = horizon_svf_comp_py(*c1.args(), 24, 1, 40000.0, 0.2, 1, 2, search_mode=1)
-    ho, so, _ = oracle.horizon_svf(c1, grid_type=1, mode=1)
+# the climb runs one work item per (cell, azimuth) and sums the sector
+# means and the SVF in a separate kernel: sub-azimuth sectors and every SVF
+# type against the oracle
+@pytest.mark.parametrize("num_hori,rf,svf_type", [(24, 1, 2), (12, 3, 1), (8, 5, 0), (40, 1, 2)])
+def test_c1_climb_mode(c1, num_hori, rf, svf_type):
+    print(f"case c1 climb nhori={num_hori} rf={rf} svf={svf_type}", flush=True)
+    st = {}
+    h, s = horizon_svf_comp_py(*c1.args(), num_hori, 1, 40000.0, 0.2, rf, svf_type,
+                               search_mode=1, stats=st)
+    ho, so, sto = oracle.horizon_svf(c1, num_hori=num_hori, grid_type=1, refine_factor=rf,
+                                     svf_type=svf_type, mode=1)
     _report("c1 climb", h, ho)
     assert np.array_equal(h, ho)
     assert np.max(np.abs(s - so)) <= SVF_TOL
+    assert st["num_rays"] == sto.num_rays
 
 
 @pytest.mark.parametrize("kernel,block,refill",
