@@ -34,7 +34,7 @@ class HzOptions(ctypes.Structure):
                 ("trace_kernel", ctypes.c_int), ("refill", ctypes.c_int),
                 ("bvh_morton", ctypes.c_int), ("queue_chunk", ctypes.c_int),
                 ("bvh_width", ctypes.c_int), ("prologue", ctypes.c_int),
-                ("reserved", ctypes.c_int * 4)]
+                ("whole_cells", ctypes.c_int), ("reserved", ctypes.c_int * 3)]
 
 
 class HzStats(ctypes.Structure):
@@ -158,7 +158,10 @@ def default_options(**kw) -> HzOptions:
     default); `prologue` also takes "host" / "device"."""
     o = HzOptions()
     lib().hz_default_options(ctypes.byref(o))
+    fields = {f for f, _ in HzOptions._fields_ if f != "reserved"}
     for k, v in kw.items():
+        if k not in fields:
+            raise TypeError(f"unknown option {k!r}")
         if v is None:
             continue
         if k == "prologue" and isinstance(v, str):

@@ -194,6 +194,7 @@ void hz_default_options(hz_options* o) {
     o->queue_chunk = 0;
     o->bvh_width = 2;
     o->prologue = HZ_PROLOGUE_HOST;
+    o->whole_cells = 0;
 }
 
 void hz_domain_destroy(hz_domain* d) {
@@ -579,7 +580,8 @@ int hz_domain_run(hz_domain* d, int num_hori, double dist_search, int refine_fac
     // search spill sized for the items' lanes
     double* hitem = nullptr;
     const long long items = (long long)d->ncell * num_hori * refine_factor;
-    if (o.search_mode != HZ_MODE_WALK && o.trace_kernel == 0 && items < (1LL << 30)) {
+    if (o.search_mode != HZ_MODE_WALK && o.trace_kernel == 0 && !o.whole_cells &&
+        items < (1LL << 30)) {
         if ((size_t)items > d->hitem_doubles) {
             if (d->d_hitem) RCHK(hipFree(d->d_hitem));
             d->d_hitem = nullptr;

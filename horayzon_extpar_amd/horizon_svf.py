@@ -101,7 +101,7 @@ def horizon_svf_comp_py(clon, clat, hsurf, vlon, vlat, cells_of_vertex, num_hori
                         grid_type, dist_search, ray_org_elev, refine_factor, svf_type, *,
                         search_mode=None, leaf_size=None, block_size=None,
                         collect_counters=None, trace_kernel=None, refill=None,
-                        bvh_morton=None, bvh_width=None, prologue=None,
+                        bvh_morton=None, bvh_width=None, prologue=None, whole_cells=None,
                         return_slope_aspect=False, stats=None):
     """Compute the terrain horizon [deg] and sky view factor [-].
 
@@ -126,13 +126,13 @@ def horizon_svf_comp_py(clon, clat, hsurf, vlon, vlat, cells_of_vertex, num_hori
     opt = default_options(search_mode=search_mode, leaf_size=leaf_size,
                           block_size=block_size, collect_counters=collect_counters,
                           trace_kernel=trace_kernel, refill=refill, bvh_morton=bvh_morton,
-                          bvh_width=bvh_width, prologue=prologue)
+                          bvh_width=bvh_width, prologue=prologue, whole_cells=whole_cells)
     if return_slope_aspect:
         with ResidentDomain(clon, clat, hsurf, vlon, vlat, cells_of_vertex, grid_type,
                             ray_org_elev, prologue=opt.prologue) as dom:
             kw = {f: getattr(opt, f) for f in ("search_mode", "leaf_size", "block_size",
                                                "collect_counters", "trace_kernel", "refill",
-                                               "bvh_morton", "bvh_width")}
+                                               "bvh_morton", "bvh_width", "whole_cells")}
             st_run = dom.run(num_hori, dist_search, refine_factor, svf_type, **kw)
             horizon, skyview = dom.fetch()
             slope, aspect = dom.slope_aspect()

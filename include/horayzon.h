@@ -58,7 +58,10 @@ typedef struct hz_options {
                              HZ_PROLOGUE_DEVICE: on the GPU from the raw ICON arrays, with
                              correctly rounded sin / cos (glibc's differ in the last bit on
                              ~0.2% of arguments); the ENU origin is still the host's */
-    int reserved[4];
+    int whole_cells;      /* lattice searches (HZ_MODE_CLIMB, HZ_MODE_LEARNED) in the
+                             persistent kernel: 0 (default) = one work item per (cell,
+                             azimuth), 1 = one per cell (as the walk); same results */
+    int reserved[3];
 } hz_options;
 
 typedef struct hz_stats {

@@ -39,6 +39,11 @@ def test_exports_every_declared_symbol():
 def test_default_options():
     o = _lib.default_options()
     assert o.search_mode == 0 and o.leaf_size >= 1 and o.cell_order == 1
+    assert o.whole_cells == 0 and o.prologue == 0
+    # hz_options is 16 ints (include/horayzon.h)
+    assert ctypes.sizeof(_lib.HzOptions) == 16 * 4
+    with pytest.raises(TypeError):
+        _lib.default_options(no_such_option=1)
 
 
 @pytest.mark.parametrize("grid_type", [0, 1])

@@ -5,6 +5,8 @@
   differences) evaluated with the same BF16 roundings; inference gives every
   row the same result whatever the batch size and offset.
 * The native trainer lowers the loss.
+* The lattice searches give the same rays, horizons and SVF with one
+  (cell, azimuth) work item at a time (default) and one cell at a time.
 * search_mode=2 (lattice search started at the predicted horizon) equals the
   CPU oracle's mode 2 bit for bit given the device's predictions; with a
   constant prediction below 0 deg it is the plain climb, rays included; with
@@ -88,6 +90,25 @@ def test_training_reduces_loss():
     loss = tr.fit(epochs=20, batch=512, lr=3e-3)
     print(f"loss {loss[0]:.3f} -> {loss[-1]:.3f}", flush=True)
     assert loss[-1] < 0.2 * loss[0]
+
+
+def test_work_items_equal_whole_cells(c1):
+    # the lattice searches run one (cell, azimuth) work item at a time by
+    # default; one cell per work item (whole_cells=1) gives the same rays,
+    # horizons and SVF
+    print("case (cell, azimuth) items vs whole cells", flush=True)
+    pred = learned.Predictor.from_file(learned.DEFAULT_WEIGHTS)
+    for dom_in in (c1, _small(23)):
+        with ResidentDomain(*dom_in.args(), grid_type=1, ray_org_elev=0.2) as rd:
+            rd.set_predictor(pred)
+            for mode, nh, rf, svf in ((1, 24, 1, 2), (1, 12, 3, 0), (2, 24, 1, 2), (2, 8, 3, 1)):
+                out = []
+                for whole in (0, 1):
+                    st = rd.run(nh, 40000.0, rf, svf, search_mode=mode, whole_cells=whole)
+                    out.append((st["num_rays"],) + rd.fetch())
+                assert out[0][0] == out[1][0], (mode, nh, rf)
+                assert np.array_equal(out[0][1], out[1][1]), (mode, nh, rf)
+                assert np.array_equal(out[0][2], out[1][2]), (mode, nh, rf)
 
 
 def _const_predictor(value):
