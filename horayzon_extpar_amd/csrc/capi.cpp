@@ -125,6 +125,10 @@ struct hz_domain {
     LbvhWork work;
     DeviceBvh bvh;
     hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    // learned search: the predictor stage runs on its own stream beside the
+    // LBVH build and the cell ordering (created on first use)
+    hipStream_t pstream = nullptr;
+    hipEvent_t pev[3] = {nullptr, nullptr, nullptr};  // fork, predictor start, join
     bool ran = false;
 };
 
@@ -200,6 +204,7 @@ void hz_default_options(hz_options* o) {
 void hz_domain_destroy(hz_domain* d) {
     if (!d) return;
     (void)hipSetDevice(d->device);
+    if (d->pstream) (void)hipStreamSynchronize(d->pstream);
     if (d->stream) (void)hipStreamSynchronize(d->stream);
     void* bufs[] = {d->d_vert,   d->d_tri,     d->d_cells,   d->d_order, d->d_nodes,
                     d->d_tris,   d->d_horizon, d->d_skyview, d->d_state,
@@ -212,6 +217,9 @@ void hz_domain_destroy(hz_domain* d) {
     d->raster.free_all();
     for (auto& e : d->ev)
         if (e) (void)hipEventDestroy(e);
+    for (auto& e : d->pev)
+        if (e) (void)hipEventDestroy(e);
+    if (d->pstream) (void)hipStreamDestroy(d->pstream);
     if (d->stream) (void)hipStreamDestroy(d->stream);
     delete d;
 }
@@ -410,8 +418,8 @@ int hz_sincos(const double* x, long long n, double* s, double* c, int on_device)
                         std::string(#call) + ": " + hipGetErrorString(e_));            \
     } while (0)
 
-// Raster, descriptors and predictor on all cells of d (enqueued on d's stream).
-static int predict_cells(hz_domain* d, std::string& err) {
+// Raster, descriptors and predictor on all cells of d (enqueued on st).
+static int predict_cells(hz_domain* d, hipStream_t st, std::string& err) {
     const int n = d->ncell;
     if (!d->d_desc) {
         if (hipMalloc(&d->d_desc, (size_t)n * kIn * sizeof(uint16_t)) != hipSuccess ||
@@ -423,11 +431,11 @@ static int predict_cells(hz_domain* d, std::string& err) {
     const float* b = d->bounds_xy;
     double area = std::max(1.0, (double)(b[2] - b[0]) * (double)(b[3] - b[1]));
     float pix = (float)std::sqrt(area / std::max<long long>(1, d->ntri));
-    if (raster_build(d->raster, d->d_vert, d->d_tri, (int)d->ntri, b, pix, d->stream, err))
+    if (raster_build(d->raster, d->d_vert, d->d_tri, (int)d->ntri, b, pix, st, err))
         return HZ_EDEVICE;
-    descriptors(d->raster, d->d_cells, n, d->d_desc, d->stream);
+    descriptors(d->raster, d->d_cells, n, d->d_desc, st);
     if (d->predictor) {
-        mlp_infer(d->predictor->net, d->d_desc, n, d->d_pred, d->stream);
+        mlp_infer(d->predictor->net, d->d_desc, n, d->d_pred, st);
         d->have_pred = true;
     }
     if (hipGetLastError() != hipSuccess) {
@@ -452,7 +460,7 @@ int hz_domain_descriptors(hz_domain* d, unsigned short* desc) {
     std::string err;
     const hz_predictor* keep = d->predictor;
     d->predictor = nullptr;  // descriptors only
-    int rc = predict_cells(d, err);
+    int rc = predict_cells(d, d->stream, err);
     d->predictor = keep;
     if (rc) return fail(rc, err);
     RCHK(hipMemcpyAsync(desc, d->d_desc, (size_t)d->ncell * kIn * sizeof(uint16_t),
@@ -534,6 +542,11 @@ int hz_domain_run(hz_domain* d, int num_hori, double dist_search, int refine_fac
     if (leaf < 1 || leaf > kMaxLeafSize) return fail(HZ_EINVAL, "leaf_size must be in [1, 8]");
     int width = o.bvh_width ? o.bvh_width : 2;
     if (width != 2 && width != 4) return fail(HZ_EINVAL, "bvh_width must be 2 or 4");
+    if (o.bvh_morton < 0 || o.bvh_morton > 1) return fail(HZ_EINVAL, "bvh_morton must be 0 or 1");
+    if (o.trace_kernel < 0 || o.trace_kernel > 1)
+        return fail(HZ_EINVAL, "trace_kernel must be 0 or 1");
+    int refill = o.refill ? o.refill : 40;
+    if (refill < 1 || refill > 64) return fail(HZ_EINVAL, "refill must be in [1, 64]");
     if (o.trace_kernel == 1) width = 2;  // the lane-per-cell kernel walks binary trees
     bool verbose = o.verbose || verbose_env();
     RCHK(hipSetDevice(d->device));
@@ -547,9 +560,34 @@ int hz_domain_run(hz_domain* d, int num_hori, double dist_search, int refine_fac
     std::string err;
     hipStream_t st = d->stream;
     RCHK(hipEventRecord(d->ev[0], st));
+    const bool learned = o.search_mode == HZ_MODE_LEARNED;
+    if (learned) {
+        // the predictor stage (raster, descriptors, network) reads only the
+        // scene and the frames: it runs on a stream of its own beside the
+        // LBVH build and the cell ordering, and the trace waits for both
+        if (!d->pstream) {
+            hipStream_t s2 = nullptr;
+            hipEvent_t e[3] = {nullptr, nullptr, nullptr};
+            hipError_t e_ = hipStreamCreateWithFlags(&s2, hipStreamNonBlocking);
+            for (int i = 0; i < 3 && e_ == hipSuccess; i++) e_ = hipEventCreate(&e[i]);
+            if (e_ != hipSuccess) {
+                for (auto& x : e)
+                    if (x) (void)hipEventDestroy(x);
+                if (s2) (void)hipStreamDestroy(s2);
+                return fail(HZ_EDEVICE, std::string("predictor stream: ") + hipGetErrorString(e_));
+            }
+            d->pstream = s2;
+            for (int i = 0; i < 3; i++) d->pev[i] = e[i];
+        }
+        RCHK(hipEventRecord(d->pev[0], st));
+        RCHK(hipStreamWaitEvent(d->pstream, d->pev[0], 0));
+        RCHK(hipEventRecord(d->pev[1], d->pstream));
+        rc = predict_cells(d, d->pstream, err);
+        if (rc) return fail(rc, err);
+        RCHK(hipEventRecord(d->pev[2], d->pstream));
+    }
     d->bvh.nodes = d->d_nodes;
     d->bvh.tris = d->d_tris;
-    if (o.bvh_morton < 0 || o.bvh_morton > 1) return fail(HZ_EINVAL, "bvh_morton must be 0 or 1");
     rc = lbvh_build(d->work, d->d_vert, d->d_tri, (int)d->ntri, leaf, o.bvh_morton, width, st,
                     d->bvh, o.collect_counters != 0, err);
     if (rc) return fail(rc == -3 ? HZ_ENOMEM : (rc == -1 ? HZ_EINVAL : HZ_EDEVICE), err);
@@ -564,17 +602,12 @@ int hz_domain_run(hz_domain* d, int num_hori, double dist_search, int refine_fac
     WalkParams p = make_walk_params(num_hori, refine_factor, svf_type, dist_search, d->ncell);
     p.elev_cs = d->d_elev_cs;
     p.sin_h = reinterpret_cast<const double*>(d->d_elev_cs + (kMaxLattice + 1));
-    if (o.search_mode == HZ_MODE_LEARNED) {
-        rc = predict_cells(d, err);
-        if (rc) return fail(rc, err);
+    if (learned) {
+        RCHK(hipStreamWaitEvent(st, d->pev[2], 0));
         p.pred = d->d_pred;
         p.pred_stride = kOutPad;
     }
     RCHK(hipEventRecord(d->ev[4], st));
-    if (o.trace_kernel < 0 || o.trace_kernel > 1)
-        return fail(HZ_EINVAL, "trace_kernel must be 0 or 1");
-    int refill = o.refill ? o.refill : 40;
-    if (refill < 1 || refill > 64) return fail(HZ_EINVAL, "refill must be in [1, 64]");
     // the lattice searches (climb, learned start) run one work item per cell
     // and azimuth in the persistent kernel: a per-item result buffer, and the
     // search spill sized for the items' lanes
@@ -615,8 +648,8 @@ int hz_domain_run(hz_domain* d, int num_hori, double dist_search, int refine_fac
     float ms_bvh = 0, ms_ord = 0, ms_trace = 0;
     RCHK(hipEventElapsedTime(&ms_bvh, d->ev[0], d->ev[1]));
     RCHK(hipEventElapsedTime(&ms_ord, d->ev[1], d->ev[2]));
-    float ms_pred = 0;
-    RCHK(hipEventElapsedTime(&ms_pred, d->ev[2], d->ev[4]));
+    float ms_pred = 0;  // the predictor stage on its own stream
+    if (learned) RCHK(hipEventElapsedTime(&ms_pred, d->pev[1], d->pev[2]));
     RCHK(hipEventElapsedTime(&ms_trace, d->ev[4], d->ev[3]));
     double t1 = now_ms();
     d->last_nhori = num_hori;
