@@ -615,22 +615,34 @@ int hz_domain_run(hz_domain* d, int num_hori, double dist_search, int refine_fac
     const long long items = (long long)d->ncell * num_hori * refine_factor;
     if (o.search_mode != HZ_MODE_WALK && o.trace_kernel == 0 && !o.whole_cells &&
         items < (1LL << 30)) {
+        // without the memory for them the search runs a cell per work item
+        // (same results) in the buffers the domain already holds
+        bool room = true;
         if ((size_t)items > d->hitem_doubles) {
             if (d->d_hitem) RCHK(hipFree(d->d_hitem));
             d->d_hitem = nullptr;
             d->hitem_doubles = 0;
-            RCHK(hipMalloc(&d->d_hitem, (size_t)items * sizeof(double)));
-            d->hitem_doubles = (size_t)items;
+            if (hipMalloc(&d->d_hitem, (size_t)items * sizeof(double)) == hipSuccess) {
+                d->hitem_doubles = (size_t)items;
+            } else {
+                (void)hipGetLastError();
+                d->d_hitem = nullptr;
+                room = false;
+            }
         }
         const size_t need = trace_spill_doubles((int)items);
-        if (need > d->spill_doubles) {
-            RCHK(hipFree(d->d_spill));
-            d->d_spill = nullptr;
-            d->spill_doubles = 0;
-            RCHK(hipMalloc(&d->d_spill, need * sizeof(double)));
-            d->spill_doubles = need;
+        if (room && need > d->spill_doubles) {
+            double* sp = nullptr;
+            if (hipMalloc(&sp, need * sizeof(double)) == hipSuccess) {
+                RCHK(hipFree(d->d_spill));
+                d->d_spill = sp;
+                d->spill_doubles = need;
+            } else {
+                (void)hipGetLastError();
+                room = false;
+            }
         }
-        hitem = d->d_hitem;
+        if (room) hitem = d->d_hitem;
     }
     rc = launch_trace(d->bvh, d->d_cells, order, p, o.search_mode, o.trace_kernel, block, refill,
                       o.collect_counters != 0, d->d_horizon, d->d_skyview, d->d_state, d->d_spill,
