@@ -611,9 +611,9 @@ __global__ __launch_bounds__(BLOCK) void k_trace(const BvhNode* __restrict__ nod
 // cell with its sector means and SVF written out) and issue their next query;
 // lanes whose cell is finished take the next cell from the wave's chunk of
 // the Morton-ordered cell list (with ITEMS: the next (cell, azimuth) item).
-// Chunks of 64 cells come from one atomic
-// counter per XCD, each over its own contiguous eighth of that list; a wave
-// whose eighth is used up moves on to the next one.  This keeps SIMD lanes
+// A wave takes as many cells as its lanes ask for (at most 64) from one
+// atomic counter per XCD, each over its own contiguous eighth of that list;
+// a wave whose eighth is used up moves on to the next one.  This keeps SIMD lanes
 // busy although searches need 1 to ~200 queries per azimuth, and keeps each
 // XCD's L2 working set spatially compact.
 // ---------------------------------------------------------------------------
@@ -708,12 +708,16 @@ __global__ __launch_bounds__(BLOCK, 4) void k_trace_pt(const BvhNode* __restrict
                     while (tries < 8) {
                         const int rlo = (int)(((long long)cur * N) >> 3);
                         const int rhi = (int)(((long long)(cur + 1) * N) >> 3);
+                        // only as many as the wave's lanes ask for now (at most
+                        // `chunk`): a wave holds no spare cells when the queues
+                        // run dry, which keeps the end of the launch short
+                        const int take = min(chunk, __popcll(need));
                         int old = 0;
-                        if (lane == 0) old = atomicAdd(&queue[cur * kQueueStride], chunk);
+                        if (lane == 0) old = atomicAdd(&queue[cur * kQueueStride], take);
                         old = __shfl(old, 0, 64);
                         if (rlo + old < rhi) {
                             qb = rlo + old;
-                            qe = min(qb + chunk, rhi);
+                            qe = min(qb + take, rhi);
                             break;
                         }
                         cur = (cur + 1) & 7;

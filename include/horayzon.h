@@ -47,7 +47,8 @@ typedef struct hz_options {
     int refill;           /* persistent kernel: refill lanes once fewer than this many of
                              the 64 lanes of a wavefront hold a live query (1..64; 0 = default) */
     int bvh_morton;       /* LBVH codes: 0 = 3D in a cube of the largest extent, 1 = 2D x/y */
-    int queue_chunk;      /* persistent kernel: cells taken per work-queue fetch (0 = 64) */
+    int queue_chunk;      /* persistent kernel: most cells a wavefront takes per work-queue
+                             fetch (it takes as many as its lanes ask for; 0 = 64) */
     int bvh_width;        /* traversal tree: 2 = binary nodes with full-precision boxes
                              (default), 4 = four-wide nodes with 8-bit quantized child boxes
                              (half the node fetches, same results); the lane-per-cell kernel
