@@ -34,7 +34,8 @@ class HzOptions(ctypes.Structure):
                 ("trace_kernel", ctypes.c_int), ("refill", ctypes.c_int),
                 ("bvh_morton", ctypes.c_int), ("queue_chunk", ctypes.c_int),
                 ("bvh_width", ctypes.c_int), ("prologue", ctypes.c_int),
-                ("whole_cells", ctypes.c_int), ("reserved", ctypes.c_int * 3)]
+                ("whole_cells", ctypes.c_int), ("certify", ctypes.c_int),
+                ("reserved", ctypes.c_int * 2)]
 
 
 class HzStats(ctypes.Structure):
@@ -48,7 +49,9 @@ class HzStats(ctypes.Structure):
                 ("t_total_ms", ctypes.c_double), ("lon0", ctypes.c_double),
                 ("lat0", ctypes.c_double), ("t_predict_ms", ctypes.c_double),
                 ("refill_frac", ctypes.c_double), ("lanes_per_iter", ctypes.c_double),
-                ("reserved", ctypes.c_double * 5)]
+                ("cert_walk_cells", ctypes.c_double), ("t_certify_ms", ctypes.c_double),
+                ("t_cert_walk_ms", ctypes.c_double), ("t_border_walk_ms", ctypes.c_double),
+                ("spec_rays", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}

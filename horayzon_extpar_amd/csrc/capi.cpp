@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <cstddef>
 #include <cstring>
 #include <limits>
 #include <vector>
@@ -124,7 +125,22 @@ struct hz_domain {
     bool have_pred = false;
     LbvhWork work;
     DeviceBvh bvh;
-    hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    // certified lattice searches: mesh-boundary index, per-item bounds, the
+    // cells handed to the walk, and the cell elevation range
+    CertIndex cert;
+    int* d_kb = nullptr;
+    size_t kb_cap = 0;
+    int* d_fix = nullptr;
+    double hmin = 0.0, hmax = 0.0;
+    // certified lattice searches: cells next to mesh holes are walked first
+    hipEvent_t bev[2] = {nullptr, nullptr};  // border walk start / end
+    TraceState* d_state2 = nullptr;  // the border walk
+    TraceState* d_state3 = nullptr;  // spare trace state
+    hipEvent_t cev[4] = {nullptr, nullptr, nullptr, nullptr};  // [2]: end of the listed walk
+    int* d_ranges = nullptr;  // class counts and {first, count} position ranges
+    bool near_flags = false;  // flag bit 1 set on the cells (certified runs)
+    int* d_blist = nullptr;
+    hipEvent_t ev[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     // learned search: the predictor stage runs on its own stream beside the
     // LBVH build and the cell ordering (created on first use)
     hipStream_t pstream = nullptr;
@@ -199,6 +215,7 @@ void hz_default_options(hz_options* o) {
     o->bvh_width = 2;
     o->prologue = HZ_PROLOGUE_HOST;
     o->whole_cells = 0;
+    o->certify = 1;
 }
 
 void hz_domain_destroy(hz_domain* d) {
@@ -209,16 +226,24 @@ void hz_domain_destroy(hz_domain* d) {
     void* bufs[] = {d->d_vert,   d->d_tri,     d->d_cells,   d->d_order, d->d_nodes,
                     d->d_tris,   d->d_horizon, d->d_skyview, d->d_state,
                     d->d_spill,  d->d_elev_cs, d->d_desc,    d->d_pred,
-                    d->d_adj_off, d->d_adj,    d->d_slope,   d->d_hitem};
+                    d->d_adj_off, d->d_adj,    d->d_slope,   d->d_hitem,
+                    d->d_kb,     d->d_fix,     d->d_state2,  d->d_state3, d->d_blist,
+                    d->d_ranges};
     d->stager.release();
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     lbvh_free(d->work);
+    cert_index_free(d->cert);
     d->raster.free_all();
     for (auto& e : d->ev)
         if (e) (void)hipEventDestroy(e);
     for (auto& e : d->pev)
         if (e) (void)hipEventDestroy(e);
+    for (auto& e : d->bev)
+        if (e) (void)hipEventDestroy(e);
+    for (auto& e : d->cev)
+        if (e) (void)hipEventDestroy(e);
+
     if (d->pstream) (void)hipStreamDestroy(d->pstream);
     if (d->stream) (void)hipStreamDestroy(d->stream);
     delete d;
@@ -308,6 +333,11 @@ int hz_domain_create(const double* clon, const double* clat, const double* hsurf
     for (auto& e : d->ev) DCHK(hipEventCreate(&e));
     d->ncell = num_cell;
     d->grid_type = grid_type;
+    d->hmin = d->hmax = hsurf[0];
+    for (int i = 1; i < num_cell; i++) {
+        d->hmin = std::min(d->hmin, hsurf[i]);
+        d->hmax = std::max(d->hmax, hsurf[i]);
+    }
     if (where == HZ_PROLOGUE_DEVICE) {
         DeviceScene ds;
         rc = prologue_device(clon, clat, hsurf, vlon, vlat, cov, num_cell, num_vertex, grid_type,
@@ -548,6 +578,12 @@ int hz_domain_run(hz_domain* d, int num_hori, double dist_search, int refine_fac
     int refill = o.refill ? o.refill : 40;
     if (refill < 1 || refill > 64) return fail(HZ_EINVAL, "refill must be in [1, 64]");
     if (o.trace_kernel == 1) width = 2;  // the lane-per-cell kernel walks binary trees
+    if (o.certify < 0 || o.certify > 1) return fail(HZ_EINVAL, "certify must be 0 or 1");
+    const bool certify = o.certify == 1 && o.search_mode != HZ_MODE_WALK;
+    if (certify && (o.trace_kernel == 1 || o.whole_cells))
+        return fail(HZ_EINVAL,
+                    "certify needs (cell, azimuth) work items: persistent kernel, whole_cells 0 "
+                    "(or certify 0 for the plain lattice search)");
     bool verbose = o.verbose || verbose_env();
     RCHK(hipSetDevice(d->device));
     double t0 = now_ms();
@@ -586,28 +622,6 @@ int hz_domain_run(hz_domain* d, int num_hori, double dist_search, int refine_fac
         if (rc) return fail(rc, err);
         RCHK(hipEventRecord(d->pev[2], d->pstream));
     }
-    d->bvh.nodes = d->d_nodes;
-    d->bvh.tris = d->d_tris;
-    rc = lbvh_build(d->work, d->d_vert, d->d_tri, (int)d->ntri, leaf, o.bvh_morton, width, st,
-                    d->bvh, o.collect_counters != 0, err);
-    if (rc) return fail(rc == -3 ? HZ_ENOMEM : (rc == -1 ? HZ_EINVAL : HZ_EDEVICE), err);
-    RCHK(hipEventRecord(d->ev[1], st));
-    const int* order = nullptr;
-    if (o.cell_order) {
-        rc = order_cells(d->work, d->d_cells, d->ncell, d->d_order, st, err);
-        if (rc) return fail(HZ_EDEVICE, err);
-        order = d->d_order;
-    }
-    RCHK(hipEventRecord(d->ev[2], st));
-    WalkParams p = make_walk_params(num_hori, refine_factor, svf_type, dist_search, d->ncell);
-    p.elev_cs = d->d_elev_cs;
-    p.sin_h = reinterpret_cast<const double*>(d->d_elev_cs + (kMaxLattice + 1));
-    if (learned) {
-        RCHK(hipStreamWaitEvent(st, d->pev[2], 0));
-        p.pred = d->d_pred;
-        p.pred_stride = kOutPad;
-    }
-    RCHK(hipEventRecord(d->ev[4], st));
     // the lattice searches (climb, learned start) run one work item per cell
     // and azimuth in the persistent kernel: a per-item result buffer, and the
     // search spill sized for the items' lanes
@@ -642,27 +656,195 @@ int hz_domain_run(hz_domain* d, int num_hori, double dist_search, int refine_fac
                 room = false;
             }
         }
+        if (room && certify && (size_t)items > d->kb_cap) {
+            if (d->d_kb) RCHK(hipFree(d->d_kb));
+            if (d->d_fix) RCHK(hipFree(d->d_fix));
+            d->d_kb = nullptr;
+            d->d_fix = nullptr;
+            d->kb_cap = 0;
+            if (hipMalloc(&d->d_kb, (size_t)items * sizeof(int)) == hipSuccess &&
+                hipMalloc(&d->d_fix, (size_t)d->ncell * sizeof(int)) == hipSuccess) {
+                d->kb_cap = (size_t)items;
+            } else {
+                (void)hipGetLastError();
+                room = false;
+            }
+        }
         if (room) hitem = d->d_hitem;
     }
-    rc = launch_trace(d->bvh, d->d_cells, order, p, o.search_mode, o.trace_kernel, block, refill,
-                      o.collect_counters != 0, d->d_horizon, d->d_skyview, d->d_state, d->d_spill,
-                      hitem, st, err);
-    if (rc) return fail(HZ_EDEVICE, err);
-    RCHK(hipEventRecord(d->ev[3], st));
-    TraceCounters c;
-    RCHK(hipMemcpyAsync(&c, &d->d_state->cnt, sizeof(c), hipMemcpyDeviceToHost, st));
+    int mode = o.search_mode;
+    if (certify && !hitem) mode = HZ_MODE_WALK;  // the walk itself: the same results
+    const bool cert_run = certify && hitem;
+    const float tfar = (float)dist_search;
+    if (cert_run) {
+        // mesh-boundary index, then the cell classes (flag bit 1: a boundary
+        // point in reach) that the trace order sorts by
+        rc = cert_index_build(d->cert, d->d_cells, d->d_vert, d->ncell, d->d_tri, d->ntri,
+                              d->bounds_xy, d->hmin, d->hmax, tfar, num_hori, refine_factor,
+                              st, err);
+        if (rc) return fail(rc == -3 ? HZ_ENOMEM : HZ_EDEVICE, err);
+        if (!d->d_ranges) RCHK(hipMalloc(&d->d_ranges, 10 * sizeof(int)));
+        launch_cert_classes(d->cert, d->d_cells, d->ncell, false, d->d_ranges, st);
+        RCHK(hipGetLastError());
+        d->near_flags = true;
+    } else if (d->near_flags) {  // the plain order again (bit 0 first, then Morton)
+        if (!d->d_ranges) RCHK(hipMalloc(&d->d_ranges, 10 * sizeof(int)));
+        launch_cert_classes(d->cert, d->d_cells, d->ncell, true, d->d_ranges, st);
+        RCHK(hipGetLastError());
+        d->near_flags = false;
+    }
+    d->bvh.nodes = d->d_nodes;
+    d->bvh.tris = d->d_tris;
+    rc = lbvh_build(d->work, d->d_vert, d->d_tri, (int)d->ntri, leaf, o.bvh_morton, width, st,
+                    d->bvh, o.collect_counters != 0, err);
+    if (rc) return fail(rc == -3 ? HZ_ENOMEM : (rc == -1 ? HZ_EINVAL : HZ_EDEVICE), err);
+    RCHK(hipEventRecord(d->ev[1], st));
+    const int* order = nullptr;
+    if (o.cell_order) {
+        rc = order_cells(d->work, d->d_cells, d->ncell, d->d_order, st, err);
+        if (rc) return fail(HZ_EDEVICE, err);
+        order = d->d_order;
+    }
+    RCHK(hipEventRecord(d->ev[2], st));
+    WalkParams p = make_walk_params(num_hori, refine_factor, svf_type, dist_search, d->ncell);
+    p.elev_cs = d->d_elev_cs;
+    p.sin_h = reinterpret_cast<const double*>(d->d_elev_cs + (kMaxLattice + 1));
+    if (learned) {
+        RCHK(hipStreamWaitEvent(st, d->pev[2], 0));
+        p.pred = d->d_pred;
+        p.pred_stride = kOutPad;
+    }
+    RCHK(hipEventRecord(d->ev[4], st));
+    bool multi_state = false;  // the border walk and far items keep states of their own
+    float ms_cert = 0, ms_cwalk = 0, ms_bwalk = 0;
+    if (!cert_run) {
+        rc = launch_trace(d->bvh, d->d_cells, order, p, mode, o.trace_kernel, block, refill,
+                          o.collect_counters != 0, d->d_horizon, d->d_skyview, d->d_state,
+                          d->d_spill, mode == HZ_MODE_WALK ? nullptr : hitem, true, st, err);
+        if (rc) return fail(HZ_EDEVICE, err);
+        RCHK(hipEventRecord(d->ev[3], st));
+    } else {
+        // Three classes of cells in the trace order (d_ranges: {first, count}
+        // of each): cells next to mesh holes, cells with a mesh-boundary point
+        // in reach, the rest.  On one stream:
+        // * border cells (the longest walks: void sectors, descents to -85
+        //   deg) run the reference walk by teams of lanes;
+        // * every other cell's lattice items; the far cells' results stand
+        //   (no boundary point in reach: always certified);
+        // * the certificate keeps the near cells' lattice results where it
+        //   holds and lists the others, whose walk then runs by teams
+        //   (certified azimuths replayed uncast).
+        // The teams traverse binary trees; with four-wide trees the persistent
+        // kernel walks the listed cells and the border cells instead.
+        // (Running the teams beside the item launches on other streams was
+        // slower: the latency-bound walks lose their SIMDs to the items.)
+        multi_state = true;
+        const bool teams = width == 2;
+        if (!d->bev[0]) {
+            hipError_t e_ = hipSuccess;
+            for (int i = 0; i < 2 && e_ == hipSuccess; i++) e_ = hipEventCreate(&d->bev[i]);
+            for (int i = 0; i < 4 && e_ == hipSuccess; i++) e_ = hipEventCreate(&d->cev[i]);
+            if (e_ != hipSuccess)
+                return fail(HZ_EDEVICE, std::string("events: ") + hipGetErrorString(e_));
+        }
+        if (!d->d_state2) RCHK(hipMalloc(&d->d_state2, sizeof(TraceState)));
+        if (!d->d_state3) RCHK(hipMalloc(&d->d_state3, sizeof(TraceState)));
+        const int nb = d->cert.npts;  // cells next to mesh holes
+        constexpr int kBorderTeam = 16, kFixTeam = 4, kTeamBlocks = 2048;
+        launch_clear_state(d->d_state2, st);
+        launch_clear_state(d->d_state3, st);
+        RCHK(hipEventRecord(d->bev[0], st));
+        if (nb > 0) {
+            if (teams) {
+                WalkParams pb = p;
+                pb.range = d->d_ranges + 2;
+                rc = launch_walk_team(d->bvh, d->d_cells, order, pb, kBorderTeam,
+                                      o.collect_counters != 0, d->d_horizon, d->d_skyview,
+                                      d->d_state2, nullptr, nullptr, nullptr, nb, kTeamBlocks, st,
+                                      err);
+            } else {
+                if (!d->d_blist) RCHK(hipMalloc(&d->d_blist, (size_t)d->ncell * sizeof(int)));
+                launch_border_list(d->d_cells, order, d->ncell, d->d_blist, d->d_state2, st);
+                rc = launch_trace_fix(d->bvh, d->d_cells, order, p, block, refill,
+                                      o.collect_counters != 0, d->d_horizon, d->d_skyview,
+                                      d->d_state2, d->d_spill, nullptr, nullptr, d->d_blist, nb,
+                                      st, err);
+            }
+            if (rc) return fail(HZ_EDEVICE, err);
+        }
+        RCHK(hipEventRecord(d->bev[1], st));
+        // items of the near and far cells, then the far cells' sector means
+        WalkParams pi = p;
+        pi.range = d->d_ranges + 8;
+        rc = launch_trace(d->bvh, d->d_cells, order, pi, mode, 0, block, refill,
+                          o.collect_counters != 0, d->d_horizon, d->d_skyview, d->d_state,
+                          d->d_spill, hitem, false, st, err);
+        if (rc) return fail(HZ_EDEVICE, err);
+        WalkParams pf = p;
+        pf.range = d->d_ranges + 6;
+        launch_items_finish(hitem, order, pf, d->d_horizon, d->d_skyview, st);
+        // certificate of the near cells, then their walk
+        RCHK(hipEventRecord(d->ev[5], st));
+        WalkParams pn = p;
+        pn.range = d->d_ranges + 4;
+        launch_certify(d->cert, d->d_cells, order, hitem, pn, d->d_horizon, d->d_skyview, d->d_kb,
+                       d->d_fix, d->d_state, false, st);
+        RCHK(hipGetLastError());
+        RCHK(hipEventRecord(d->ev[6], st));
+        if (teams)
+            rc = launch_walk_team(d->bvh, d->d_cells, order, p, kFixTeam, o.collect_counters != 0,
+                                  d->d_horizon, d->d_skyview, d->d_state, hitem, d->d_kb,
+                                  d->d_fix, d->ncell, kTeamBlocks, st, err);
+        else
+            rc = launch_trace_fix(d->bvh, d->d_cells, order, p, block, refill,
+                                  o.collect_counters != 0, d->d_horizon, d->d_skyview, d->d_state,
+                                  d->d_spill, hitem, d->d_kb, d->d_fix, d->ncell, st, err);
+        if (rc) return fail(HZ_EDEVICE, err);
+        RCHK(hipEventRecord(d->cev[2], st));
+        RCHK(hipEventRecord(d->ev[3], st));
+    }
+    TraceState ts_host, ts_border, ts_far;
+    TraceCounters& c = ts_host.cnt;
+    int ranges_host[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    RCHK(hipMemcpyAsync(&ts_host, d->d_state, offsetof(TraceState, pad), hipMemcpyDeviceToHost,
+                        st));
+    if (multi_state) {
+        RCHK(hipMemcpyAsync(&ts_border, d->d_state2, offsetof(TraceState, pad),
+                            hipMemcpyDeviceToHost, st));
+        RCHK(hipMemcpyAsync(&ts_far, d->d_state3, offsetof(TraceState, pad),
+                            hipMemcpyDeviceToHost, st));
+        RCHK(hipMemcpyAsync(ranges_host, d->d_ranges, sizeof(ranges_host), hipMemcpyDeviceToHost,
+                            st));
+    }
     unsigned nodes4 = 0;
     if (d->bvh.d_num_nodes)
         RCHK(hipMemcpyAsync(&nodes4, d->bvh.d_num_nodes, sizeof(nodes4), hipMemcpyDeviceToHost,
                             st));
     RCHK(hipStreamSynchronize(st));
     if (d->bvh.d_num_nodes) d->bvh.num_nodes = (int)nodes4;
+    if (multi_state) {  // the border walk's and the far items' rays, visits and flags
+        for (const TraceState* o2 : {&ts_border, &ts_far}) {
+            c.rays += o2->cnt.rays;
+            c.nodes += o2->cnt.nodes;
+            c.tris += o2->cnt.tris;
+            c.error |= o2->cnt.error;
+            c.cyc_refill += o2->cnt.cyc_refill;
+            c.cyc_trav += o2->cnt.cyc_trav;
+            c.trav_iters += o2->cnt.trav_iters;
+            c.spec_rays += o2->cnt.spec_rays;
+        }
+    }
     float ms_bvh = 0, ms_ord = 0, ms_trace = 0;
     RCHK(hipEventElapsedTime(&ms_bvh, d->ev[0], d->ev[1]));
     RCHK(hipEventElapsedTime(&ms_ord, d->ev[1], d->ev[2]));
     float ms_pred = 0;  // the predictor stage on its own stream
     if (learned) RCHK(hipEventElapsedTime(&ms_pred, d->pev[1], d->pev[2]));
     RCHK(hipEventElapsedTime(&ms_trace, d->ev[4], d->ev[3]));
+    if (cert_run) {
+        RCHK(hipEventElapsedTime(&ms_bwalk, d->bev[0], d->bev[1]));
+        RCHK(hipEventElapsedTime(&ms_cert, d->ev[5], d->ev[6]));
+        RCHK(hipEventElapsedTime(&ms_cwalk, d->ev[6], d->cev[2]));
+    }
     double t1 = now_ms();
     d->last_nhori = num_hori;
     d->ran = c.error == 0;
@@ -689,6 +871,12 @@ int hz_domain_run(hz_domain* d, int num_hori, double dist_search, int refine_fac
         const double cyc = (double)c.cyc_refill + (double)c.cyc_trav;
         stats->refill_frac = cyc > 0 ? (double)c.cyc_refill / cyc : 0.0;
         stats->lanes_per_iter = c.trav_iters ? (double)c.nodes / (double)c.trav_iters : 0.0;
+        // cells whose walk was cast: the listed near cells and the border cells
+        stats->cert_walk_cells = cert_run ? (double)ts_host.nfix + (double)ranges_host[3] : 0.0;
+        stats->t_certify_ms = ms_cert;
+        stats->t_cert_walk_ms = ms_cwalk;
+        stats->t_border_walk_ms = ms_bwalk;
+        stats->spec_rays = (double)c.spec_rays;
         stats->t_total_ms = t1 - t0;
         stats->lon0 = d->lon0;
         stats->lat0 = d->lat0;
@@ -945,6 +1133,9 @@ int hz_batch_run(hz_batch* b, int num_hori, double dist_search, int refine_facto
         hz_default_options(&o);
     if (o.search_mode != HZ_MODE_WALK && o.search_mode != HZ_MODE_CLIMB)
         return fail(HZ_EINVAL, "batches run the walk or the climb search");
+    if (o.search_mode == HZ_MODE_CLIMB && o.certify != 0)
+        return fail(HZ_EINVAL,
+                    "batches run the plain climb only (whole cells, no certificate): pass certify 0");
     int block = o.block_size ? o.block_size : 128;
     if (block != 64 && block != 128 && block != 256)
         return fail(HZ_EINVAL, "block_size must be 64, 128 or 256");

@@ -655,9 +655,12 @@ __global__ __launch_bounds__(kBlock) void k_cell_morton(const CellFrame* __restr
     uint64_t ux = (uint64_t)fminf(qx * 16777216.0f, 16777215.0f);
     uint64_t uy = (uint64_t)fminf(qy * 16777216.0f, 16777215.0f);
     // cells next to a mesh hole (flag bit 0) sort first: their searches are
-    // the longest, and starting them first shortens the end of the launch
-    const uint64_t later = (cells[i].pad & 1) ? 0ull : 1ull;
-    keys[i] = (later << 48) | (spread32(uy) << 1) | spread32(ux);
+    // the longest, and starting them first shortens the end of the launch;
+    // then cells with a mesh-boundary point in reach (flag bit 1, set by the
+    // certificate of the lattice searches), then the rest
+    const int fl = cells[i].pad;
+    const uint64_t cls = (fl & 1) ? 0ull : ((fl & 2) ? 1ull : 2ull);
+    keys[i] = (cls << 48) | (spread32(uy) << 1) | spread32(ux);
     vals[i] = i;
 }
 
@@ -817,7 +820,7 @@ int order_cells(LbvhWork& w, const CellFrame* cells, int n, int* order, hipStrea
                        w.keys[0], w.vals[0]);
     size_t tmp = w.sort_tmp_bytes;
     HZ_HIP(rocprim::radix_sort_pairs(w.sort_tmp, tmp, w.keys[0], w.keys[1], w.vals[0], order,
-                                     (unsigned)n, 0, 49, st),
+                                     (unsigned)n, 0, 50, st),
            err);
     HZ_HIP(hipGetLastError(), err);
     return 0;

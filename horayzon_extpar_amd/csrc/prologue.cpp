@@ -123,6 +123,8 @@ WalkParams make_walk_params(int num_hori, int refine_factor, int svf_type, doubl
     p.pred = nullptr;
     p.pred_stride = 0;
     p.sin_h = nullptr;
+    p.reserve_blocks = 0;
+    p.range = nullptr;
     return p;
 }
 

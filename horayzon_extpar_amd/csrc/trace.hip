@@ -627,11 +627,17 @@ __global__ void k_clear_state(TraceState* ts) {
     for (int i = threadIdx.x; i < (int)(sizeof(TraceState) / 4); i += blockDim.x) w[i] = 0;
 }
 
+// Work queues only (the counters of the launch before are kept).
+__global__ void k_clear_queue(TraceState* ts) {
+    for (int i = threadIdx.x; i < 8 * kQueueStride; i += blockDim.x) ts->queue[i] = 0;
+}
+
 // Per-lane FP64 search state kept in global memory between queries (it is
 // needed only when a query ends), so that the traversal loop runs with the
 // registers of the traversal alone.  Field f of lane g is spill[f * lanes + g].
 // Fields S_I0.. hold two ints each: {cell, phase}, {azimuth j, lattice index},
-// {sector step, sector index}, {batch domain, unused}.
+// {sector step, sector index}, {batch domain, unused} or (MODE 3) {trace
+// position p, walk index known}.
 enum {
     S_DIR = 0, S_AX = 3, S_E = 6, S_SECTOR = 7, S_SVF = 8,
     S_I0 = 9, S_I1 = 10, S_I2 = 11, S_I3 = 12, S_FIELDS = 13
@@ -647,6 +653,14 @@ static_assert(S_FIELDS == S_FIELDS_HOST, "spill layout");
 // azimuth) pairs instead of cells, since these searches start every azimuth
 // afresh; item results go to hitem and k_items_finish forms the sector means
 // and the SVF.
+// MODE 3: the reference walk of the cells the certificate could not vouch
+// for (the positions fixlist[0 .. ts->nfix)), bit for bit: its elevation,
+// rotations and -85 deg floor test are the walk's own, and kk tracks the
+// lattice index of its elevation.  An azimuth whose lattice top hit index
+// (from hitem) and starting index both lie at or above the certified bound
+// kbuf sees monotone hits, so the walk's steps there are replayed without
+// casting; every other azimuth, and every one without a hit at 0 deg (its
+// end, and so the next start, is unknown), is cast.
 template <int MODE, bool COUNT, int BLOCK, int TREE, bool BATCH, bool ITEMS = false>
 __global__ __launch_bounds__(BLOCK, 4) void k_trace_pt(const BvhNode* __restrict__ nodes,
                                                     const float4* __restrict__ tris, int num_tri,
@@ -658,8 +672,12 @@ __global__ __launch_bounds__(BLOCK, 4) void k_trace_pt(const BvhNode* __restrict
                                                     TraceState* __restrict__ ts,
                                                     double* __restrict__ spill,
                                                     const BatchDomain* __restrict__ doms, int ndom,
-                                                    int total, double* __restrict__ hitem) {
-    static_assert(!ITEMS || (MODE != 0 && !BATCH), "items: lattice searches of one domain");
+                                                    int total, double* __restrict__ hitem,
+                                                    const int* __restrict__ kbuf,
+                                                    const int* __restrict__ fixlist) {
+    static_assert(!ITEMS || (MODE != 0 && MODE != 3 && !BATCH),
+                  "items: lattice searches of one domain");
+    static_assert(MODE != 3 || (!ITEMS && !BATCH), "the certified walk runs whole cells");
     __shared__ int stk_all[kLdsStack * BLOCK];
     // the wave's part of the stack: stk[k * BLOCK + lane] (the base is uniform)
     int* stk = stk_all + (__builtin_amdgcn_readfirstlane(threadIdx.x) & ~63);
@@ -668,7 +686,10 @@ __global__ __launch_bounds__(BLOCK, 4) void k_trace_pt(const BvhNode* __restrict
     const unsigned long long lt_mask = (1ull << lane) - 1ull;
     const int L = P.num_hori * P.refine_factor;
     // size of the work index space (the launcher keeps ncell * L below 2^31)
-    const int N = BATCH ? total : (ITEMS ? P.ncell * L : P.ncell);
+    // items: positions [ibase, ibase + npos) of the trace order (P.range)
+    const int ibase = (ITEMS && P.range) ? P.range[0] : 0;
+    const int npos = (ITEMS && P.range) ? P.range[1] : P.ncell;
+    const int N = BATCH ? total : (ITEMS ? npos * L : (MODE == 3 ? ts->nfix : P.ncell));
     const double a = P.hori_acc;
     int* queue = ts->queue;
     const size_t lanes = (size_t)gridDim.x * BLOCK;
@@ -752,9 +773,12 @@ __global__ __launch_bounds__(BLOCK, 4) void k_trace_pt(const BvhNode* __restrict
                             // 64 neighbouring cells
                             const int blk = g / (kItemCells * L);
                             const int rem = g - blk * (kItemCells * L);
-                            const int nb = min(kItemCells, P.ncell - blk * kItemCells);
+                            const int nb = min(kItemCells, npos - blk * kItemCells);
                             item_j = rem / nb;
-                            item_p = blk * kItemCells + (rem - item_j * nb);
+                            item_p = ibase + blk * kItemCells + (rem - item_j * nb);
+                            cell = order ? order[item_p] : item_p;
+                        } else if (MODE == 3) {
+                            item_p = fixlist[g];
                             cell = order ? order[item_p] : item_p;
                         } else {
                             cell = order ? order[g] : g;
@@ -780,6 +804,7 @@ __global__ __launch_bounds__(BLOCK, 4) void k_trace_pt(const BvhNode* __restrict
                         SI(S_I1) = make_int2(ITEMS ? item_j : 0, 0);
                         SI(S_I2) = make_int2(0, ITEMS ? item_p : 0);
                         if (BATCH) SI(S_I3) = make_int2(dom, 0);
+                        if (MODE == 3) SI(S_I3) = make_int2(item_p, 0);
                     }
                     qb += min(__popcll(need), avail);
                 }
@@ -789,6 +814,8 @@ __global__ __launch_bounds__(BLOCK, 4) void k_trace_pt(const BvhNode* __restrict
                 const int2 i0 = SI(S_I0), i1 = SI(S_I1), i2 = SI(S_I2);
                 const int cell = i0.x;
                 int phase = i0.y, j = i1.x, kk = i1.y, ksec = i2.x, jsec = i2.y;
+                int p3 = 0;
+                if (MODE == 3) p3 = SI(S_I3).x;
                 const CellFrame* lcells = cells;
                 double* lhorizon = horizon;
                 double* lskyview = skyview;
@@ -805,7 +832,7 @@ __global__ __launch_bounds__(BLOCK, 4) void k_trace_pt(const BvhNode* __restrict
                 D3 dir = {SV(S_DIR), SV(S_DIR + 1), SV(S_DIR + 2)};
                 D3 ax = {0.0, 0.0, 0.0};
                 double e = 0.0;
-                if (MODE == 0) {
+                if (MODE == 0 || MODE == 3) {
                     ax = {SV(S_AX), SV(S_AX + 1), SV(S_AX + 2)};
                     e = SV(S_E);
                 }
@@ -827,26 +854,59 @@ __global__ __launch_bounds__(BLOCK, 4) void k_trace_pt(const BvhNode* __restrict
                 for (int hint_round = 0;; hint_round++) {
                 issue = false;
                 while (true) {
+                    double h = 0.0;
                     if (phase == P_AZSTART) {
-                        phase = P_FIRST;
-                        if (MODE == 0) {
+                        if (MODE == 3) {
+                            // the reference walk, except where the certificate
+                            // shows the hits it would explore to be monotone:
+                            // then its steps are replayed without casting
+                            const size_t at = (size_t)j * P.ncell + p3;
+                            const int kbj = kbuf ? kbuf[at] : kCertAll;
+                            const int top = kbuf ? lattice_top(hitem[at], a) : -1;
                             ax = unit_cross(dir, n);
-                            q = dir;
+                            if (top >= 0 && kk >= kbj && top >= kbj) {
+                                if (kk <= top) {
+                                    while (kk <= top) {  // hits up to top, miss at top + 1
+                                        e += 2.0 * a;
+                                        dir = rodrigues(dir, ax, P.elev_sin_2ha, P.elev_cos_2ha);
+                                        kk++;
+                                    }
+                                    h = e - a;
+                                } else {
+                                    while (kk > top) {  // misses down to the hit at top
+                                        e -= 2.0 * a;
+                                        dir = rodrigues(dir, ax, -P.elev_sin_2ha, P.elev_cos_2ha);
+                                        kk--;
+                                    }
+                                    h = e + a;
+                                }
+                            } else {
+                                phase = P_FIRST;
+                                q = dir;
+                                issue = true;
+                                break;
+                            }
                         } else {
-                            kk = MODE == 1 ? 0
-                                           : start_index(P.pred[(size_t)cell * P.pred_stride + j]);
-                            q = lattice_dir(dir, n, P.elev_cs, kk);
+                            phase = P_FIRST;
+                            if (MODE == 0) {
+                                ax = unit_cross(dir, n);
+                                q = dir;
+                            } else {
+                                kk = MODE == 1
+                                         ? 0
+                                         : start_index(P.pred[(size_t)cell * P.pred_stride + j]);
+                                q = lattice_dir(dir, n, P.elev_cs, kk);
+                            }
+                            issue = true;
+                            break;
                         }
-                        issue = true;
-                        break;
-                    }
-                    double h;
-                    if (MODE == 0) {  // reference walk (:521-546)
+                    } else if (MODE == 0 || MODE == 3) {  // reference walk (:521-546)
                         if (phase == P_FIRST) {
                             if (last_hit) {
                                 phase = P_UP;
                                 e += 2.0 * a;
                                 dir = rodrigues(dir, ax, P.elev_sin_2ha, P.elev_cos_2ha);
+                                if (MODE == 3) kk++;
                                 q = dir;
                                 issue = true;
                                 break;
@@ -855,6 +915,7 @@ __global__ __launch_bounds__(BLOCK, 4) void k_trace_pt(const BvhNode* __restrict
                                 phase = P_DOWN;
                                 e -= 2.0 * a;
                                 dir = rodrigues(dir, ax, -P.elev_sin_2ha, P.elev_cos_2ha);
+                                if (MODE == 3) kk--;
                                 q = dir;
                                 issue = true;
                                 break;
@@ -865,6 +926,7 @@ __global__ __launch_bounds__(BLOCK, 4) void k_trace_pt(const BvhNode* __restrict
                                 if (!(e > kPi)) {
                                     e += 2.0 * a;
                                     dir = rodrigues(dir, ax, P.elev_sin_2ha, P.elev_cos_2ha);
+                                    if (MODE == 3) kk++;
                                     q = dir;
                                     issue = true;
                                     break;
@@ -876,6 +938,7 @@ __global__ __launch_bounds__(BLOCK, 4) void k_trace_pt(const BvhNode* __restrict
                             if (!last_hit && e > P.elev_thresh) {
                                 e -= 2.0 * a;
                                 dir = rodrigues(dir, ax, -P.elev_sin_2ha, P.elev_cos_2ha);
+                                if (MODE == 3) kk--;
                                 q = dir;
                                 issue = true;
                                 break;
@@ -961,6 +1024,7 @@ __global__ __launch_bounds__(BLOCK, 4) void k_trace_pt(const BvhNode* __restrict
                     SI(S_I0) = make_int2(cell, phase);
                     SI(S_I1) = make_int2(j, kk);
                     SI(S_I2) = make_int2(ksec, jsec);
+                    if (MODE == 3) SI(S_I3) = make_int2(p3, 0);
                     node = kWideBit;  // the root is read in full
                     sp = 0;
                     steps = 0;
@@ -970,7 +1034,7 @@ __global__ __launch_bounds__(BLOCK, 4) void k_trace_pt(const BvhNode* __restrict
                     SV(S_DIR + 2) = dir.z;
                     SV(S_SECTOR) = sector;
                     SV(S_SVF) = svf;
-                    if (MODE == 0) {
+                    if (MODE == 0 || MODE == 3) {
                         SV(S_AX) = ax.x;
                         SV(S_AX + 1) = ax.y;
                         SV(S_AX + 2) = ax.z;
@@ -1041,13 +1105,235 @@ __global__ __launch_bounds__(BLOCK, 4) void k_trace_pt(const BvhNode* __restrict
     }
 }
 
+// ---------------------------------------------------------------------------
+// Cooperative walk: the reference walk of listed cells, one cell per team of
+// TEAM lanes.  Within an azimuth the walk's queries lie on a run of fixed
+// rotations about the azimuth's axis, so their directions do not depend on
+// the answers: a lane reaches step m of the run by the walk's own m
+// rotations and elevation updates from the run's current state (the same
+// FP64 operations in the same order, so the same bits).  Round 0 casts the
+// azimuth's first query (lane 0) with up-steps (lanes 1 .. U) and down-steps
+// (lanes U + 1 .. TEAM - 1) beside it; later rounds cast the next TEAM steps
+// of the run the first answer chose.  The walk then takes the first answer
+// that ends its run (:521-546), or the zenith / -85 deg limit, exactly as
+// one lane would, so a run of r steps costs about r / TEAM rounds instead of
+// r dependent queries.
+// ---------------------------------------------------------------------------
+template <int TEAM, bool COUNT, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_walk_team(
+    const BvhNode* __restrict__ nodes, const float4* __restrict__ tris, int num_tri,
+    const CellFrame* __restrict__ cells, const int* __restrict__ order,
+    const int* __restrict__ list, WalkParams P, double* __restrict__ horizon,
+    double* __restrict__ skyview, TraceState* __restrict__ ts, const double* __restrict__ hitem,
+    const int* __restrict__ kbuf) {
+    static_assert(TEAM >= 2 && TEAM <= 64 && (TEAM & (TEAM - 1)) == 0, "team size");
+    __shared__ int stk_all[kLdsStack * BLOCK];
+    int* stk = stk_all + (__builtin_amdgcn_readfirstlane(threadIdx.x) & ~63);
+    const int lane = threadIdx.x & 63;
+    const int t = lane & (TEAM - 1);
+    const int tb = lane - t;
+    const unsigned long long full = TEAM == 64 ? ~0ull : ((1ull << TEAM) - 1ull);
+    constexpr int U = TEAM / 2 - 1;  // up-steps of round 0 (lanes 1 .. U)
+    constexpr int D = TEAM - 1 - U;  // down-steps of round 0 (lanes U + 1 .. TEAM - 1)
+    unsigned long long rays = 0, spec = 0;
+    unsigned nvisit = 0, ntest = 0;
+    int err = 0;
+    // cells from one counter (queue[0]): the list's entries, or the
+    // positions of P.range; every lane of a team takes the same cell
+    const int count = list ? ts->nfix : P.range[1];
+    while (true) {
+        int idx = 0;
+        if (t == 0) idx = atomicAdd(&ts->queue[0], 1);
+        idx = __shfl(idx, tb, 64);
+        if (idx >= count) break;
+        const int p = list ? list[idx] : P.range[0] + idx;
+        const int cell = order ? order[p] : p;
+        const CellFrame& f = cells[cell];
+        const D3 n = {f.n[0], f.n[1], f.n[2]};
+        const D3 north = {f.north[0], f.north[1], f.north[2]};
+        const float o0 = f.org[0], o1 = f.org[1], o2 = f.org[2];
+        const double a = P.hori_acc;
+        const int L = P.num_hori * P.refine_factor;
+        const bool have = num_tri > 0;
+        D3 dir = rodrigues(north, n, P.shift_sin, P.shift_cos);
+        double e = 0.0;
+        int kk = 0;  // lattice index of e (for the certified replay)
+        double sector = 0.0, svf = 0.0;
+        int ksec = 0, jsec = 0;
+        for (int j = 0; j < L; j++) {
+            const D3 ax = unit_cross(dir, n);
+            double h;
+            bool done = false;
+            if (kbuf) {  // certified azimuth: the walk's steps replayed uncast
+                const size_t at = (size_t)j * P.ncell + p;
+                const int kbj = kbuf[at];
+                const int top = lattice_top(hitem[at], a);
+                if (top >= 0 && kk >= kbj && top >= kbj) {
+                    if (kk <= top) {
+                        while (kk <= top) {
+                            e += 2.0 * a;
+                            dir = rodrigues(dir, ax, P.elev_sin_2ha, P.elev_cos_2ha);
+                            kk++;
+                        }
+                        h = e - a;
+                    } else {
+                        while (kk > top) {
+                            e -= 2.0 * a;
+                            dir = rodrigues(dir, ax, -P.elev_sin_2ha, P.elev_cos_2ha);
+                            kk--;
+                        }
+                        h = e + a;
+                    }
+                    done = true;
+                }
+            }
+            int sgn = 0;  // the run's direction once known (+1 up, -1 down)
+            for (int round = 0; !done; round++) {
+                // this lane's step: round 0 = start / up / down lanes
+                int m, ls;
+                if (round == 0) {
+                    if (t == 0) {
+                        m = 0;
+                        ls = 0;
+                    } else if (t <= U) {
+                        m = t;
+                        ls = 1;
+                    } else {
+                        m = t - U;
+                        ls = -1;
+                    }
+                } else {
+                    m = t + 1;
+                    ls = sgn;
+                }
+                D3 d = dir;
+                double ee = e;
+                bool valid = true;
+                for (int q = 0; q < m; q++) {
+                    if (ls > 0) {
+                        if (ee > kPi) {  // the walk stops past the zenith
+                            valid = false;
+                            break;
+                        }
+                        ee += 2.0 * a;
+                        d = rodrigues(d, ax, P.elev_sin_2ha, P.elev_cos_2ha);
+                    } else {
+                        if (!(ee > P.elev_thresh)) {  // the -85 deg floor
+                            valid = false;
+                            break;
+                        }
+                        ee -= 2.0 * a;
+                        d = rodrigues(d, ax, -P.elev_sin_2ha, P.elev_cos_2ha);
+                    }
+                }
+                bool hit = false;
+                if (valid && have) {
+                    const Ray r = make_ray(o0, o1, o2, (float)d.x, (float)d.y, (float)d.z, P.tfar,
+                                           P.tfar_box);
+                    hit = occluded<COUNT, BLOCK>(nodes, tris, stk, r, nvisit, ntest, err);
+                }
+                const unsigned long long hb = (__ballot(hit) >> tb) & full;
+                const unsigned long long vb = (__ballot(valid) >> tb) & full;
+                const int cast = __popcll(vb);
+                // steps of the run this round, bit r-1 = step r
+                unsigned long long vm, am;
+                int R;
+                if (round == 0) {
+                    sgn = (hb & 1ull) ? 1 : -1;
+                    if (sgn > 0) {
+                        R = U;
+                        vm = (vb >> 1) & ((1ull << U) - 1ull);
+                        am = ~(hb >> 1) & vm;  // misses end an up run
+                    } else {
+                        R = D;
+                        vm = (vb >> (U + 1)) & ((1ull << D) - 1ull);
+                        am = (hb >> (U + 1)) & vm;  // hits end a down run
+                    }
+                } else {
+                    R = TEAM;
+                    vm = vb;
+                    am = sgn > 0 ? (~hb & vb) : (hb & vb);
+                }
+                const unsigned long long rmask = R == 64 ? ~0ull : ((1ull << R) - 1ull);
+                const unsigned long long inval = ~vm & rmask;
+                const int fi = inval ? __builtin_ctzll(inval) : R;  // first invalid step - 1
+                const int fa = am ? __builtin_ctzll(am) : R;        // first ending answer - 1
+                int steps, used;
+                if (fa < fi) {
+                    steps = fa + 1;
+                    used = fa + 1;
+                    done = true;
+                } else if (fi < R) {
+                    steps = fi;
+                    used = fi;
+                    done = true;
+                    if (sgn > 0) err |= 2;  // a ray past the zenith still hits
+                } else {
+                    steps = R;
+                    used = R;
+                }
+                if (round == 0) used += 1;  // the azimuth's first query
+                if (t == 0) {
+                    rays += (unsigned long long)used;
+                    spec += (unsigned long long)(cast - used);
+                }
+                for (int q = 0; q < steps; q++) {  // the run's state after `steps` steps
+                    if (sgn > 0) {
+                        e += 2.0 * a;
+                        dir = rodrigues(dir, ax, P.elev_sin_2ha, P.elev_cos_2ha);
+                        kk++;
+                    } else {
+                        e -= 2.0 * a;
+                        dir = rodrigues(dir, ax, -P.elev_sin_2ha, P.elev_cos_2ha);
+                        kk--;
+                    }
+                }
+                h = sgn > 0 ? e - a : e + a;
+            }
+            if (h < 0.0) h = 0.0;  // (:737-741)
+            sector += h;
+            if (++ksec == P.refine_factor) {  // (:744-751)
+                if (t == 0)
+                    horizon[(size_t)jsec * P.ncell + cell] =
+                        ((sector / kPi) * 180.0) / (double)P.refine_factor;
+                sector = 0.0;
+                ksec = 0;
+                jsec++;
+            }
+            const double sn = sin_horizon(P, h);  // (:333-358)
+            if (P.svf_type == 0)
+                svf += (1.0 - sn);
+            else if (P.svf_type == 1)
+                svf += (1.0 - (sn * sn));
+            else
+                svf += (1.0 - (sn * sn * sn));
+            dir = rodrigues(dir, n, -P.azim_sin, P.azim_cos);  // (:550-551)
+        }
+        if (t == 0) skyview[cell] = svf / (double)L;
+    }
+    const unsigned long long wr = wave_sum(rays), wsp = wave_sum(spec);
+    const unsigned long long wn = COUNT ? wave_sum(nvisit) : 0ull;
+    const unsigned long long wt = COUNT ? wave_sum(ntest) : 0ull;
+    const int werr = __any(err != 0);
+    if (lane == 0) {
+        if (wr) atomicAdd(&ts->cnt.rays, wr);
+        if (wsp) atomicAdd(&ts->cnt.spec_rays, wsp);
+        if (COUNT) {
+            atomicAdd(&ts->cnt.nodes, wn);
+            atomicAdd(&ts->cnt.tris, wt);
+        }
+        if (werr) atomicOr(&ts->cnt.error, 1ull);
+    }
+}
+
 // Sector means and SVF of the item results of one domain, cell by cell in
 // the order of the cell-by-cell search (:736-754).
 __global__ void k_items_finish(const double* __restrict__ hitem, const int* __restrict__ order,
                                WalkParams P, double* __restrict__ horizon,
                                double* __restrict__ skyview) {
-    const int p = blockIdx.x * 256 + threadIdx.x;
-    if (p >= P.ncell) return;
+    const int pr = blockIdx.x * 256 + threadIdx.x;
+    if (pr >= (P.range ? P.range[1] : P.ncell)) return;
+    const int p = (P.range ? P.range[0] : 0) + pr;
     const int cell = order ? order[p] : p;
     const int L = P.num_hori * P.refine_factor;
     double sector = 0.0, svf = 0.0;
@@ -1074,7 +1360,7 @@ __global__ void k_items_finish(const double* __restrict__ hitem, const int* __re
 }
 
 template <int MODE, bool COUNT, int BLOCK, int TREE, bool BATCH, bool ITEMS = false>
-int pt_grid(int ncell) {
+int pt_grid(int ncell, int reserve = 0) {
     static const int occ_grid = []() {
         int dev = 0, cus = 0, per = 0;
         if (hipGetDevice(&dev) != hipSuccess) return 0;
@@ -1088,31 +1374,47 @@ int pt_grid(int ncell) {
         return std::max(1, per) * std::max(1, cus);
     }();
     int want = (ncell + BLOCK - 1) / BLOCK;
-    int g = occ_grid > 0 ? std::min(occ_grid, want) : want;
+    int g = occ_grid > 0 ? std::min(std::max(8, occ_grid - reserve), want) : want;
     return std::max(8, (g + 7) / 8 * 8);
 }
 
 template <int MODE, bool COUNT, int BLOCK, int TREE>
 void launch_pt(const DeviceBvh& bvh, const CellFrame* cells, const int* order,
                const WalkParams& p, int refill, double* horizon, double* skyview,
-               TraceState* ts, double* spill, double* hitem, hipStream_t st) {
+               TraceState* ts, double* spill, double* hitem, bool finish, hipStream_t st) {
     if constexpr (MODE != 0) {
         if (hitem) {
             const int items = p.ncell * p.num_hori * p.refine_factor;
             hipLaunchKernelGGL((k_trace_pt<MODE, COUNT, BLOCK, TREE, false, true>),
-                               dim3(pt_grid<MODE, COUNT, BLOCK, TREE, false, true>(items)),
+                               dim3(pt_grid<MODE, COUNT, BLOCK, TREE, false, true>(
+                                   items, p.reserve_blocks)),
                                dim3(BLOCK), 0, st, bvh.nodes, bvh.tris, bvh.num_tri, cells, order,
                                p, refill, kChunk, horizon, skyview, ts, spill, nullptr, 0, 0,
-                               hitem);
-            hipLaunchKernelGGL(k_items_finish, dim3((p.ncell + 255) / 256), dim3(256), 0, st,
-                               hitem, order, p, horizon, skyview);
+                               hitem, nullptr, nullptr);
+            if (finish)
+                hipLaunchKernelGGL(k_items_finish, dim3((p.ncell + 255) / 256), dim3(256), 0, st,
+                                   hitem, order, p, horizon, skyview);
             return;
         }
     }
     hipLaunchKernelGGL((k_trace_pt<MODE, COUNT, BLOCK, TREE, false>),
                        dim3(pt_grid<MODE, COUNT, BLOCK, TREE, false>(p.ncell)), dim3(BLOCK), 0,
                        st, bvh.nodes, bvh.tris, bvh.num_tri, cells, order, p, refill, kChunk,
-                       horizon, skyview, ts, spill, nullptr, 0, 0, nullptr);
+                       horizon, skyview, ts, spill, nullptr, 0, 0, nullptr, nullptr, nullptr);
+}
+
+// The walk of the cells the certificate listed (MODE 3), after the lattice
+// item kernel and k_certify on the same stream.
+template <bool COUNT, int BLOCK, int TREE>
+void launch_pt_fix(const DeviceBvh& bvh, const CellFrame* cells, const int* order,
+                   const WalkParams& p, int refill, double* horizon, double* skyview,
+                   TraceState* ts, double* spill, double* hitem, const int* kb,
+                   const int* fixlist, int max_work, hipStream_t st) {
+    hipLaunchKernelGGL(k_clear_queue, dim3(1), dim3(256), 0, st, ts);
+    hipLaunchKernelGGL((k_trace_pt<3, COUNT, BLOCK, TREE, false>),
+                       dim3(pt_grid<3, COUNT, BLOCK, TREE, false>(max_work)), dim3(BLOCK), 0, st,
+                       bvh.nodes, bvh.tris, bvh.num_tri, cells, order, p, refill, kChunk,
+                       horizon, skyview, ts, spill, nullptr, 0, 0, hitem, kb, fixlist);
 }
 
 template <int MODE, int BLOCK, int TREE>
@@ -1121,7 +1423,7 @@ void launch_pt_batch(const BatchDomain* doms, int ndom, int total, const WalkPar
     hipLaunchKernelGGL((k_trace_pt<MODE, false, BLOCK, TREE, true>),
                        dim3(pt_grid<MODE, false, BLOCK, TREE, true>(total)), dim3(BLOCK), 0, st,
                        nullptr, nullptr, 0, nullptr, nullptr, p, refill, chunk, nullptr, nullptr,
-                       ts, spill, doms, ndom, total, nullptr);
+                       ts, spill, doms, ndom, total, nullptr, nullptr, nullptr);
 }
 
 inline int tree_kind(bool multi, int width) { return width == 4 ? 2 : (multi ? 1 : 0); }
@@ -1129,18 +1431,19 @@ inline int tree_kind(bool multi, int width) { return width == 4 ? 2 : (multi ? 1
 template <int MODE, bool COUNT, int BLOCK>
 void launch_one(const DeviceBvh& bvh, const CellFrame* cells, const int* order,
                 const WalkParams& p, int kernel, bool multi, int refill, double* horizon,
-                double* skyview, TraceState* ts, double* spill, double* hitem, hipStream_t st) {
+                double* skyview, TraceState* ts, double* spill, double* hitem, bool finish,
+                hipStream_t st) {
     if (kernel == 0) {
         const int tk = tree_kind(multi, bvh.width);
         if (tk == 2)
             launch_pt<MODE, COUNT, BLOCK, 2>(bvh, cells, order, p, refill, horizon, skyview, ts,
-                                             spill, hitem, st);
+                                             spill, hitem, finish, st);
         else if (tk == 1)
             launch_pt<MODE, COUNT, BLOCK, 1>(bvh, cells, order, p, refill, horizon, skyview, ts,
-                                             spill, hitem, st);
+                                             spill, hitem, finish, st);
         else
             launch_pt<MODE, COUNT, BLOCK, 0>(bvh, cells, order, p, refill, horizon, skyview, ts,
-                                             spill, hitem, st);
+                                             spill, hitem, finish, st);
         return;
     }
     if constexpr (MODE == 2) {
@@ -1156,16 +1459,33 @@ void launch_one(const DeviceBvh& bvh, const CellFrame* cells, const int* order,
 template <int MODE, bool COUNT>
 void launch_block(int block, const DeviceBvh& bvh, const CellFrame* cells, const int* order,
                   const WalkParams& p, int kernel, bool multi, int refill, double* horizon,
-                  double* skyview, TraceState* ts, double* spill, double* hitem, hipStream_t st) {
+                  double* skyview, TraceState* ts, double* spill, double* hitem, bool finish,
+                  hipStream_t st) {
     if (block == 64)
         launch_one<MODE, COUNT, 64>(bvh, cells, order, p, kernel, multi, refill, horizon, skyview,
-                                    ts, spill, hitem, st);
+                                    ts, spill, hitem, finish, st);
     else if (block == 256)
         launch_one<MODE, COUNT, 256>(bvh, cells, order, p, kernel, multi, refill, horizon,
-                                     skyview, ts, spill, hitem, st);
+                                     skyview, ts, spill, hitem, finish, st);
     else
         launch_one<MODE, COUNT, 128>(bvh, cells, order, p, kernel, multi, refill, horizon,
-                                     skyview, ts, spill, hitem, st);
+                                     skyview, ts, spill, hitem, finish, st);
+}
+
+template <bool COUNT, int BLOCK>
+void fix_tree(int tk, const DeviceBvh& bvh, const CellFrame* cells, const int* order,
+              const WalkParams& p, int refill, double* horizon, double* skyview, TraceState* ts,
+              double* spill, double* hitem, const int* kb, const int* fixlist, int max_work,
+              hipStream_t st) {
+    if (tk == 2)
+        launch_pt_fix<COUNT, BLOCK, 2>(bvh, cells, order, p, refill, horizon, skyview, ts, spill,
+                                       hitem, kb, fixlist, max_work, st);
+    else if (tk == 1)
+        launch_pt_fix<COUNT, BLOCK, 1>(bvh, cells, order, p, refill, horizon, skyview, ts, spill,
+                                       hitem, kb, fixlist, max_work, st);
+    else
+        launch_pt_fix<COUNT, BLOCK, 0>(bvh, cells, order, p, refill, horizon, skyview, ts, spill,
+                                       hitem, kb, fixlist, max_work, st);
 }
 
 template <int MODE, int BLOCK>
@@ -1241,10 +1561,90 @@ int launch_trace_batch(const BatchDomain* doms, int ndom, int total, bool multi,
     return 0;
 }
 
+template <int TEAM, bool COUNT>
+void launch_team(const DeviceBvh& bvh, const CellFrame* cells, const int* order,
+                 const WalkParams& p, double* horizon, double* skyview, TraceState* ts,
+                 const double* hitem, const int* kb, const int* list, int max_cells,
+                 int grid_blocks, hipStream_t st) {
+    constexpr int B = 128;
+    const long long lanes = (long long)max_cells * TEAM;
+    const int grid = (int)std::max<long long>(
+        1, std::min<long long>(grid_blocks, (lanes + B - 1) / B));
+    hipLaunchKernelGGL(k_clear_queue, dim3(1), dim3(256), 0, st, ts);
+    hipLaunchKernelGGL((k_walk_team<TEAM, COUNT, B>), dim3(grid), dim3(B), 0, st, bvh.nodes,
+                       bvh.tris, bvh.num_tri, cells, order, list, p, horizon, skyview, ts, hitem,
+                       kb);
+}
+
+template <bool COUNT>
+void team_size(int team, const DeviceBvh& bvh, const CellFrame* cells, const int* order,
+               const WalkParams& p, double* horizon, double* skyview, TraceState* ts,
+               const double* hitem, const int* kb, const int* list, int max_cells,
+               int grid_blocks, hipStream_t st) {
+    switch (team) {
+        case 2: launch_team<2, COUNT>(bvh, cells, order, p, horizon, skyview, ts, hitem, kb, list, max_cells, grid_blocks, st); break;
+        case 4: launch_team<4, COUNT>(bvh, cells, order, p, horizon, skyview, ts, hitem, kb, list, max_cells, grid_blocks, st); break;
+        case 8: launch_team<8, COUNT>(bvh, cells, order, p, horizon, skyview, ts, hitem, kb, list, max_cells, grid_blocks, st); break;
+        case 32: launch_team<32, COUNT>(bvh, cells, order, p, horizon, skyview, ts, hitem, kb, list, max_cells, grid_blocks, st); break;
+        case 64: launch_team<64, COUNT>(bvh, cells, order, p, horizon, skyview, ts, hitem, kb, list, max_cells, grid_blocks, st); break;
+        default: launch_team<16, COUNT>(bvh, cells, order, p, horizon, skyview, ts, hitem, kb, list, max_cells, grid_blocks, st); break;
+    }
+}
+
+void launch_items_finish(const double* hitem, const int* order, const WalkParams& p,
+                         double* horizon, double* skyview, hipStream_t st) {
+    hipLaunchKernelGGL(k_items_finish, dim3((p.ncell + 255) / 256), dim3(256), 0, st, hitem, order,
+                       p, horizon, skyview);
+}
+
+void launch_clear_state(TraceState* ts, hipStream_t st) {
+    hipLaunchKernelGGL(k_clear_state, dim3(1), dim3(256), 0, st, ts);
+}
+
+int launch_walk_team(const DeviceBvh& bvh, const CellFrame* cells, const int* order,
+                     const WalkParams& p, int team, bool counters, double* horizon,
+                     double* skyview, TraceState* ts, const double* hitem, const int* kb,
+                     const int* list, int max_cells, int grid_blocks, hipStream_t st,
+                     std::string& err) {
+    if (bvh.width != 2) {
+        err = "the cooperative walk traverses binary trees only";
+        return -1;
+    }
+    max_cells = std::max(1, std::min(max_cells, p.ncell));
+    if (counters)
+        team_size<true>(team, bvh, cells, order, p, horizon, skyview, ts, hitem, kb, list,
+                        max_cells, grid_blocks, st);
+    else
+        team_size<false>(team, bvh, cells, order, p, horizon, skyview, ts, hitem, kb, list,
+                         max_cells, grid_blocks, st);
+    HZ_HIP(hipGetLastError(), err);
+    return 0;
+}
+
+int launch_trace_fix(const DeviceBvh& bvh, const CellFrame* cells, const int* order,
+                     const WalkParams& p, int block, int refill, bool counters, double* horizon,
+                     double* skyview, TraceState* ts, double* spill, double* hitem, const int* kb,
+                     const int* fixlist, int max_work, hipStream_t st, std::string& err) {
+    if (refill < 1 || refill > 64) refill = 40;
+    const int tk = tree_kind(bvh.leaf_size > 1, bvh.width);
+    max_work = std::max(1, std::min(max_work, p.ncell));
+#define HZ_FIX(C, B) \
+    fix_tree<C, B>(tk, bvh, cells, order, p, refill, horizon, skyview, ts, spill, hitem, kb, \
+                   fixlist, max_work, st)
+    if (counters) {
+        if (block == 64) HZ_FIX(true, 64); else if (block == 256) HZ_FIX(true, 256); else HZ_FIX(true, 128);
+    } else {
+        if (block == 64) HZ_FIX(false, 64); else if (block == 256) HZ_FIX(false, 256); else HZ_FIX(false, 128);
+    }
+#undef HZ_FIX
+    HZ_HIP(hipGetLastError(), err);
+    return 0;
+}
+
 int launch_trace(const DeviceBvh& bvh, const CellFrame* cells, const int* order,
                  const WalkParams& p, int mode, int kernel, int block, int refill, bool counters,
                  double* horizon, double* skyview, TraceState* ts, double* spill, double* hitem,
-                 hipStream_t st, std::string& err) {
+                 bool items_finish, hipStream_t st, std::string& err) {
     const bool multi = bvh.leaf_size > 1;
     if (refill < 1 || refill > 64) refill = 40;
     hipLaunchKernelGGL(k_clear_state, dim3(1), dim3(256), 0, st, ts);
@@ -1260,10 +1660,10 @@ int launch_trace(const DeviceBvh& bvh, const CellFrame* cells, const int* order,
     do {                                                                                      \
         if (counters)                                                                         \
             launch_block<M, true>(block, bvh, cells, order, p, kernel, multi, refill, horizon,   \
-                                  skyview, ts, spill, hitem, st);                             \
+                                  skyview, ts, spill, hitem, items_finish, st);               \
         else                                                                                  \
             launch_block<M, false>(block, bvh, cells, order, p, kernel, multi, refill, horizon,  \
-                                   skyview, ts, spill, hitem, st);                            \
+                                   skyview, ts, spill, hitem, items_finish, st);              \
     } while (0)
     if (mode == 0)
         HZ_LAUNCH(0);

@@ -102,7 +102,7 @@ def horizon_svf_comp_py(clon, clat, hsurf, vlon, vlat, cells_of_vertex, num_hori
                         search_mode=None, leaf_size=None, block_size=None,
                         collect_counters=None, trace_kernel=None, refill=None,
                         bvh_morton=None, bvh_width=None, prologue=None, whole_cells=None,
-                        return_slope_aspect=False, stats=None):
+                        certify=None, return_slope_aspect=False, stats=None):
     """Compute the terrain horizon [deg] and sky view factor [-].
 
     Same contract as the reference binding (horizon_svf.pyx:15-124):
@@ -126,13 +126,15 @@ def horizon_svf_comp_py(clon, clat, hsurf, vlon, vlat, cells_of_vertex, num_hori
     opt = default_options(search_mode=search_mode, leaf_size=leaf_size,
                           block_size=block_size, collect_counters=collect_counters,
                           trace_kernel=trace_kernel, refill=refill, bvh_morton=bvh_morton,
-                          bvh_width=bvh_width, prologue=prologue, whole_cells=whole_cells)
+                          bvh_width=bvh_width, prologue=prologue, whole_cells=whole_cells,
+                          certify=certify)
     if return_slope_aspect:
         with ResidentDomain(clon, clat, hsurf, vlon, vlat, cells_of_vertex, grid_type,
                             ray_org_elev, prologue=opt.prologue) as dom:
             kw = {f: getattr(opt, f) for f in ("search_mode", "leaf_size", "block_size",
                                                "collect_counters", "trace_kernel", "refill",
-                                               "bvh_morton", "bvh_width", "whole_cells")}
+                                               "bvh_morton", "bvh_width", "whole_cells",
+                                               "certify")}
             st_run = dom.run(num_hori, dist_search, refine_factor, svf_type, **kw)
             horizon, skyview = dom.fetch()
             slope, aspect = dom.slope_aspect()

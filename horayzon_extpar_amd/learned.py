@@ -167,7 +167,7 @@ def domain_samples(dom, grid_type: int = 1, dist_search: float = 40000.0):
     lattice search, deg) of every cell of a synthetic domain."""
     with ResidentDomain(*dom.args(), grid_type=grid_type, ray_org_elev=0.2) as rd:
         X = rd.descriptors()
-        rd.run(24, dist_search, 1, 2, search_mode=1)
+        rd.run(24, dist_search, 1, 2, search_mode=1, certify=0)
         h, _ = rd.fetch()
     return X, np.ascontiguousarray(h.T, np.float32)
 

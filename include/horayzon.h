@@ -62,7 +62,15 @@ typedef struct hz_options {
     int whole_cells;      /* lattice searches (HZ_MODE_CLIMB, HZ_MODE_LEARNED) in the
                              persistent kernel: 0 (default) = one work item per (cell,
                              azimuth), 1 = one per cell (as the walk); same results */
-    int reserved[3];
+    int certify;          /* lattice searches (HZ_MODE_CLIMB, HZ_MODE_LEARNED): 1 (default) =
+                             equal to the reference walk: a geometric certificate shows, per
+                             cell and azimuth, where hits are monotone in elevation over what
+                             the walk explores (no mesh-boundary point below those rays), and
+                             the walk's own steps are cast wherever it does not; 0 = the plain
+                             lattice search, which differs from the walk where its rays pass
+                             through mesh holes.  Needs (cell, azimuth) work items (persistent
+                             kernel, whole_cells 0); without memory for them the walk runs */
+    int reserved[2];
 } hz_options;
 
 typedef struct hz_stats {
@@ -87,7 +95,15 @@ typedef struct hz_stats {
                                     time spent refilling lanes (search state, next query) */
     double lanes_per_iter;       /* collect_counters, persistent kernel: live lanes (of 64) per
                                     traversal iteration */
-    double reserved[5];
+    double cert_walk_cells;      /* certified lattice searches: cells whose walk was cast in
+                                    part (the rest took the lattice results unchanged) */
+    double t_certify_ms;         /* certified lattice searches: certificate kernel (HIP events) */
+    double t_cert_walk_ms;       /* certified lattice searches: the walk of the listed cells
+                                    (from the certificate to the end of the search) */
+    double t_border_walk_ms;     /* certified lattice searches: the walk of the cells next to
+                                    mesh holes, run from the start beside the lattice items */
+    double spec_rays;            /* queries the cooperative walk cast ahead and did not use
+                                    (not in num_rays) */
 } hz_stats;
 
 /*
@@ -198,7 +214,8 @@ int hz_domain_slope_aspect(hz_domain* dom, double* slope, double* aspect);
  * of maximum terrain elevation angle, 0.5 .. 40 km), its output the horizon
  * per azimuth [deg].  HZ_MODE_LEARNED starts each azimuth's lattice search at
  * the predicted elevation: it climbs while the query hits and descends while
- * it misses, so a wrong prediction costs queries, never accuracy.
+ * it misses, so a wrong prediction costs queries, never accuracy; with the
+ * default option certify = 1 its results equal the reference walk's.
  *
  * params layout (hz_predictor_num_params() floats): w1[256][192], b1[256],
  * w2[256][256], b2[256], w3[24][256], b3[24].

@@ -104,7 +104,7 @@ def test_work_items_equal_whole_cells(c1):
             for mode, nh, rf, svf in ((1, 24, 1, 2), (1, 12, 3, 0), (2, 24, 1, 2), (2, 8, 3, 1)):
                 out = []
                 for whole in (0, 1):
-                    st = rd.run(nh, 40000.0, rf, svf, search_mode=mode, whole_cells=whole)
+                    st = rd.run(nh, 40000.0, rf, svf, search_mode=mode, whole_cells=whole, certify=0)
                     out.append((st["num_rays"],) + rd.fetch())
                 assert out[0][0] == out[1][0], (mode, nh, rf)
                 assert np.array_equal(out[0][1], out[1][1]), (mode, nh, rf)
@@ -126,13 +126,13 @@ def test_constant_predictions_reproduce_climb():
     dom = _small(21)
     low, high = _const_predictor(-3.0), _const_predictor(12.0)
     with ResidentDomain(*dom.args(), grid_type=1) as rd:
-        st1 = rd.run(24, 40000.0, 1, 2, search_mode=1)
+        st1 = rd.run(24, 40000.0, 1, 2, search_mode=1, certify=0)
         h1, s1 = rd.fetch()
         rd.set_predictor(low)
-        stl = rd.run(24, 40000.0, 1, 2, search_mode=2)
+        stl = rd.run(24, 40000.0, 1, 2, search_mode=2, certify=0)
         hl, sl = rd.fetch()
         rd.set_predictor(high)
-        sth = rd.run(24, 40000.0, 1, 2, search_mode=2)
+        sth = rd.run(24, 40000.0, 1, 2, search_mode=2, certify=0)
         hh, sh = rd.fetch()
         P = rd.predictions()
     assert np.array_equal(hl, h1) and np.array_equal(sl, s1)
@@ -159,10 +159,10 @@ def test_trained_predictor_saves_queries():
     dom = cfg(34)  # held-out terrain
     with ResidentDomain(*dom.args(), grid_type=1) as rd:
         rd.set_predictor(pred)
-        st2 = rd.run(24, 40000.0, 1, 2, search_mode=2)
+        st2 = rd.run(24, 40000.0, 1, 2, search_mode=2, certify=0)
         h2, s2 = rd.fetch()
         P = rd.predictions()
-        st1 = rd.run(24, 40000.0, 1, 2, search_mode=1)
+        st1 = rd.run(24, 40000.0, 1, 2, search_mode=1, certify=0)
         h1, s1 = rd.fetch()
         st0 = rd.run(24, 40000.0, 1, 2, search_mode=0)
         h0, s0 = rd.fetch()

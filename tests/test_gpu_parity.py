@@ -47,15 +47,15 @@ def test_c1_refine_factor_10(c1):
     assert np.max(np.abs(s - so)) <= SVF_TOL
 
 
-# the climb runs one work item per (cell, azimuth) and sums the sector
-# means and the SVF in a separate kernel: sub-azimuth sectors and every SVF
-# type against the oracle
+# the plain climb (no certificate) runs one work item per (cell, azimuth)
+# and sums the sector means and the SVF in a separate kernel: sub-azimuth
+# sectors and every SVF type against the oracle
 @pytest.mark.parametrize("num_hori,rf,svf_type", [(24, 1, 2), (12, 3, 1), (8, 5, 0), (40, 1, 2)])
 def test_c1_climb_mode(c1, num_hori, rf, svf_type):
     print(f"case c1 climb nhori={num_hori} rf={rf} svf={svf_type}", flush=True)
     st = {}
     h, s = horizon_svf_comp_py(*c1.args(), num_hori, 1, 40000.0, 0.2, rf, svf_type,
-                               search_mode=1, stats=st)
+                               search_mode=1, certify=0, stats=st)
     ho, so, sto = oracle.horizon_svf(c1, num_hori=num_hori, grid_type=1, refine_factor=rf,
                                      svf_type=svf_type, mode=1)
     _report("c1 climb", h, ho)
