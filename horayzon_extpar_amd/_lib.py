@@ -51,7 +51,7 @@ class HzStats(ctypes.Structure):
                 ("refill_frac", ctypes.c_double), ("lanes_per_iter", ctypes.c_double),
                 ("cert_walk_cells", ctypes.c_double), ("t_certify_ms", ctypes.c_double),
                 ("t_cert_walk_ms", ctypes.c_double), ("t_border_walk_ms", ctypes.c_double),
-                ("spec_rays", ctypes.c_double)]
+                ("spec_rays", ctypes.c_double), ("t_cert_lattice_ms", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}

@@ -128,18 +128,18 @@ struct hz_domain {
     // certified lattice searches: mesh-boundary index, per-item bounds, the
     // cells handed to the walk, and the cell elevation range
     CertIndex cert;
-    int* d_kb = nullptr;
-    size_t kb_cap = 0;
-    int* d_fix = nullptr;
     double hmin = 0.0, hmax = 0.0;
     // certified lattice searches: cells next to mesh holes are walked first
-    hipEvent_t bev[2] = {nullptr, nullptr};  // border walk start / end
+    hipEvent_t bev[2] = {nullptr, nullptr};  // start of the three launches, border walk done
     TraceState* d_state2 = nullptr;  // the border walk
-    TraceState* d_state3 = nullptr;  // spare trace state
-    hipEvent_t cev[4] = {nullptr, nullptr, nullptr, nullptr};  // [2]: end of the listed walk
+    TraceState* d_state3 = nullptr;  // the far cells' items
+    hipEvent_t cev[4] = {nullptr, nullptr, nullptr, nullptr};  // near walk / far items done
     int* d_ranges = nullptr;  // class counts and {first, count} position ranges
     bool near_flags = false;  // flag bit 1 set on the cells (certified runs)
-    int* d_blist = nullptr;
+    hipStream_t xstream = nullptr;  // far cells' lattice items
+    hipStream_t ystream = nullptr;  // border cells' team walk
+    double* d_spill_x = nullptr;    // spill area of the far cells' items
+    size_t spill_x_doubles = 0;
     hipEvent_t ev[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     // learned search: the predictor stage runs on its own stream beside the
     // LBVH build and the cell ordering (created on first use)
@@ -222,13 +222,14 @@ void hz_domain_destroy(hz_domain* d) {
     if (!d) return;
     (void)hipSetDevice(d->device);
     if (d->pstream) (void)hipStreamSynchronize(d->pstream);
+    if (d->xstream) (void)hipStreamSynchronize(d->xstream);
+    if (d->ystream) (void)hipStreamSynchronize(d->ystream);
     if (d->stream) (void)hipStreamSynchronize(d->stream);
     void* bufs[] = {d->d_vert,   d->d_tri,     d->d_cells,   d->d_order, d->d_nodes,
                     d->d_tris,   d->d_horizon, d->d_skyview, d->d_state,
                     d->d_spill,  d->d_elev_cs, d->d_desc,    d->d_pred,
                     d->d_adj_off, d->d_adj,    d->d_slope,   d->d_hitem,
-                    d->d_kb,     d->d_fix,     d->d_state2,  d->d_state3, d->d_blist,
-                    d->d_ranges};
+                    d->d_state2, d->d_state3, d->d_ranges, d->d_spill_x};
     d->stager.release();
     for (void* b : bufs)
         if (b) (void)hipFree(b);
@@ -243,6 +244,8 @@ void hz_domain_destroy(hz_domain* d) {
         if (e) (void)hipEventDestroy(e);
     for (auto& e : d->cev)
         if (e) (void)hipEventDestroy(e);
+    if (d->xstream) (void)hipStreamDestroy(d->xstream);
+    if (d->ystream) (void)hipStreamDestroy(d->ystream);
 
     if (d->pstream) (void)hipStreamDestroy(d->pstream);
     if (d->stream) (void)hipStreamDestroy(d->stream);
@@ -656,20 +659,6 @@ int hz_domain_run(hz_domain* d, int num_hori, double dist_search, int refine_fac
                 room = false;
             }
         }
-        if (room && certify && (size_t)items > d->kb_cap) {
-            if (d->d_kb) RCHK(hipFree(d->d_kb));
-            if (d->d_fix) RCHK(hipFree(d->d_fix));
-            d->d_kb = nullptr;
-            d->d_fix = nullptr;
-            d->kb_cap = 0;
-            if (hipMalloc(&d->d_kb, (size_t)items * sizeof(int)) == hipSuccess &&
-                hipMalloc(&d->d_fix, (size_t)d->ncell * sizeof(int)) == hipSuccess) {
-                d->kb_cap = (size_t)items;
-            } else {
-                (void)hipGetLastError();
-                room = false;
-            }
-        }
         if (room) hitem = d->d_hitem;
     }
     int mode = o.search_mode;
@@ -680,12 +669,13 @@ int hz_domain_run(hz_domain* d, int num_hori, double dist_search, int refine_fac
         // mesh-boundary index, then the cell classes (flag bit 1: a boundary
         // point in reach) that the trace order sorts by
         rc = cert_index_build(d->cert, d->d_cells, d->d_vert, d->ncell, d->d_tri, d->ntri,
-                              d->bounds_xy, d->hmin, d->hmax, tfar, num_hori, refine_factor,
-                              st, err);
+                              d->bounds_xy, d->hmin, d->hmax, tfar, st, err);
         if (rc) return fail(rc == -3 ? HZ_ENOMEM : HZ_EDEVICE, err);
         if (!d->d_ranges) RCHK(hipMalloc(&d->d_ranges, 10 * sizeof(int)));
+        RCHK(hipEventRecord(d->ev[5], st));
         launch_cert_classes(d->cert, d->d_cells, d->ncell, false, d->d_ranges, st);
         RCHK(hipGetLastError());
+        RCHK(hipEventRecord(d->ev[6], st));
         d->near_flags = true;
     } else if (d->near_flags) {  // the plain order again (bit 0 first, then Morton)
         if (!d->d_ranges) RCHK(hipMalloc(&d->d_ranges, 10 * sizeof(int)));
@@ -716,7 +706,7 @@ int hz_domain_run(hz_domain* d, int num_hori, double dist_search, int refine_fac
     }
     RCHK(hipEventRecord(d->ev[4], st));
     bool multi_state = false;  // the border walk and far items keep states of their own
-    float ms_cert = 0, ms_cwalk = 0, ms_bwalk = 0;
+    float ms_cert = 0, ms_cwalk = 0, ms_bwalk = 0, ms_lattice = 0;
     if (!cert_run) {
         rc = launch_trace(d->bvh, d->d_cells, order, p, mode, o.trace_kernel, block, refill,
                           o.collect_counters != 0, d->d_horizon, d->d_skyview, d->d_state,
@@ -726,81 +716,77 @@ int hz_domain_run(hz_domain* d, int num_hori, double dist_search, int refine_fac
     } else {
         // Three classes of cells in the trace order (d_ranges: {first, count}
         // of each): cells next to mesh holes, cells with a mesh-boundary point
-        // in reach, the rest.  On one stream:
+        // in reach, the rest.  Three launches run side by side:
         // * border cells (the longest walks: void sectors, descents to -85
-        //   deg) run the reference walk by teams of lanes;
-        // * every other cell's lattice items; the far cells' results stand
-        //   (no boundary point in reach: always certified);
-        // * the certificate keeps the near cells' lattice results where it
-        //   holds and lists the others, whose walk then runs by teams
-        //   (certified azimuths replayed uncast).
-        // The teams traverse binary trees; with four-wide trees the persistent
-        // kernel walks the listed cells and the border cells instead.
-        // (Running the teams beside the item launches on other streams was
-        // slower: the latency-bound walks lose their SIMDs to the items.)
+        //   deg) run the reference walk by teams of lanes that cast its next
+        //   steps together (binary trees; with four-wide trees the
+        //   persistent walk runs them first, on the main stream);
+        // * the far cells run the lattice search (no boundary point in reach:
+        //   their hits are monotone in elevation, so it returns the walk's
+        //   values), on a second stream;
+        // * the other near cells run the reference walk (persistent kernel)
+        //   on the main stream.
+        // Side by side, the walk's latency-bound tail and the border teams
+        // overlap the lattice items: C2 learned 21.5 ms against 31.6 ms for
+        // the three in sequence.
         multi_state = true;
-        const bool teams = width == 2;
         if (!d->bev[0]) {
             hipError_t e_ = hipSuccess;
             for (int i = 0; i < 2 && e_ == hipSuccess; i++) e_ = hipEventCreate(&d->bev[i]);
             for (int i = 0; i < 4 && e_ == hipSuccess; i++) e_ = hipEventCreate(&d->cev[i]);
+            if (e_ == hipSuccess) e_ = hipStreamCreateWithFlags(&d->xstream, hipStreamNonBlocking);
+            if (e_ == hipSuccess) e_ = hipStreamCreateWithFlags(&d->ystream, hipStreamNonBlocking);
             if (e_ != hipSuccess)
-                return fail(HZ_EDEVICE, std::string("events: ") + hipGetErrorString(e_));
+                return fail(HZ_EDEVICE, std::string("certified search streams: ") +
+                                            hipGetErrorString(e_));
         }
         if (!d->d_state2) RCHK(hipMalloc(&d->d_state2, sizeof(TraceState)));
         if (!d->d_state3) RCHK(hipMalloc(&d->d_state3, sizeof(TraceState)));
+        if (d->spill_x_doubles < d->spill_doubles) {  // the far items' spill area
+            if (d->d_spill_x) RCHK(hipFree(d->d_spill_x));
+            d->d_spill_x = nullptr;
+            d->spill_x_doubles = 0;
+            RCHK(hipMalloc(&d->d_spill_x, d->spill_doubles * sizeof(double)));
+            d->spill_x_doubles = d->spill_doubles;
+        }
         const int nb = d->cert.npts;  // cells next to mesh holes
-        constexpr int kBorderTeam = 16, kFixTeam = 4, kTeamBlocks = 2048;
+        const bool teams = width == 2;
+        constexpr int kBorderTeam = 64, kTeamBlocks = 2048;
         launch_clear_state(d->d_state2, st);
-        launch_clear_state(d->d_state3, st);
-        RCHK(hipEventRecord(d->bev[0], st));
-        if (nb > 0) {
-            if (teams) {
-                WalkParams pb = p;
-                pb.range = d->d_ranges + 2;
-                rc = launch_walk_team(d->bvh, d->d_cells, order, pb, kBorderTeam,
-                                      o.collect_counters != 0, d->d_horizon, d->d_skyview,
-                                      d->d_state2, nullptr, nullptr, nullptr, nb, kTeamBlocks, st,
-                                      err);
-            } else {
-                if (!d->d_blist) RCHK(hipMalloc(&d->d_blist, (size_t)d->ncell * sizeof(int)));
-                launch_border_list(d->d_cells, order, d->ncell, d->d_blist, d->d_state2, st);
-                rc = launch_trace_fix(d->bvh, d->d_cells, order, p, block, refill,
-                                      o.collect_counters != 0, d->d_horizon, d->d_skyview,
-                                      d->d_state2, d->d_spill, nullptr, nullptr, d->d_blist, nb,
-                                      st, err);
-            }
+        WalkParams pb = p, pn = p, pf = p;
+        pb.range = d->d_ranges + 2;
+        pn.range = d->d_ranges + 4;
+        pf.range = d->d_ranges + 6;
+        if (!teams && nb > 0) {
+            rc = launch_trace(d->bvh, d->d_cells, order, pb, HZ_MODE_WALK, 0, block, refill,
+                              o.collect_counters != 0, d->d_horizon, d->d_skyview, d->d_state2,
+                              d->d_spill, nullptr, true, st, err);
             if (rc) return fail(HZ_EDEVICE, err);
         }
-        RCHK(hipEventRecord(d->bev[1], st));
-        // items of the near and far cells, then the far cells' sector means
-        WalkParams pi = p;
-        pi.range = d->d_ranges + 8;
-        rc = launch_trace(d->bvh, d->d_cells, order, pi, mode, 0, block, refill,
+        RCHK(hipEventRecord(d->bev[0], st));
+        if (teams) {
+            RCHK(hipStreamWaitEvent(d->ystream, d->bev[0], 0));
+            if (nb > 0) {
+                rc = launch_walk_team(d->bvh, d->d_cells, order, pb, kBorderTeam,
+                                      o.collect_counters != 0, d->d_horizon, d->d_skyview,
+                                      d->d_state2, nb, kTeamBlocks, d->ystream, err);
+                if (rc) return fail(HZ_EDEVICE, err);
+            }
+            RCHK(hipEventRecord(d->bev[1], d->ystream));
+        }
+        RCHK(hipStreamWaitEvent(d->xstream, d->bev[0], 0));
+        rc = launch_trace(d->bvh, d->d_cells, order, pf, mode, 0, block, refill,
+                          o.collect_counters != 0, d->d_horizon, d->d_skyview, d->d_state3,
+                          d->d_spill_x, hitem, true, d->xstream, err);
+        if (rc) return fail(HZ_EDEVICE, err);
+        RCHK(hipEventRecord(d->cev[1], d->xstream));
+        rc = launch_trace(d->bvh, d->d_cells, order, pn, HZ_MODE_WALK, 0, block, refill,
                           o.collect_counters != 0, d->d_horizon, d->d_skyview, d->d_state,
-                          d->d_spill, hitem, false, st, err);
+                          d->d_spill, nullptr, true, st, err);
         if (rc) return fail(HZ_EDEVICE, err);
-        WalkParams pf = p;
-        pf.range = d->d_ranges + 6;
-        launch_items_finish(hitem, order, pf, d->d_horizon, d->d_skyview, st);
-        // certificate of the near cells, then their walk
-        RCHK(hipEventRecord(d->ev[5], st));
-        WalkParams pn = p;
-        pn.range = d->d_ranges + 4;
-        launch_certify(d->cert, d->d_cells, order, hitem, pn, d->d_horizon, d->d_skyview, d->d_kb,
-                       d->d_fix, d->d_state, false, st);
-        RCHK(hipGetLastError());
-        RCHK(hipEventRecord(d->ev[6], st));
-        if (teams)
-            rc = launch_walk_team(d->bvh, d->d_cells, order, p, kFixTeam, o.collect_counters != 0,
-                                  d->d_horizon, d->d_skyview, d->d_state, hitem, d->d_kb,
-                                  d->d_fix, d->ncell, kTeamBlocks, st, err);
-        else
-            rc = launch_trace_fix(d->bvh, d->d_cells, order, p, block, refill,
-                                  o.collect_counters != 0, d->d_horizon, d->d_skyview, d->d_state,
-                                  d->d_spill, hitem, d->d_kb, d->d_fix, d->ncell, st, err);
-        if (rc) return fail(HZ_EDEVICE, err);
-        RCHK(hipEventRecord(d->cev[2], st));
+        RCHK(hipEventRecord(d->cev[0], st));
+        RCHK(hipStreamWaitEvent(st, d->cev[1], 0));
+        if (teams) RCHK(hipStreamWaitEvent(st, d->bev[1], 0));
         RCHK(hipEventRecord(d->ev[3], st));
     }
     TraceState ts_host, ts_border, ts_far;
@@ -841,9 +827,10 @@ int hz_domain_run(hz_domain* d, int num_hori, double dist_search, int refine_fac
     if (learned) RCHK(hipEventElapsedTime(&ms_pred, d->pev[1], d->pev[2]));
     RCHK(hipEventElapsedTime(&ms_trace, d->ev[4], d->ev[3]));
     if (cert_run) {
-        RCHK(hipEventElapsedTime(&ms_bwalk, d->bev[0], d->bev[1]));
+        if (width == 2) RCHK(hipEventElapsedTime(&ms_bwalk, d->bev[0], d->bev[1]));
+        RCHK(hipEventElapsedTime(&ms_cwalk, d->bev[0], d->cev[0]));
+        RCHK(hipEventElapsedTime(&ms_lattice, d->bev[0], d->cev[1]));
         RCHK(hipEventElapsedTime(&ms_cert, d->ev[5], d->ev[6]));
-        RCHK(hipEventElapsedTime(&ms_cwalk, d->ev[6], d->cev[2]));
     }
     double t1 = now_ms();
     d->last_nhori = num_hori;
@@ -871,11 +858,12 @@ int hz_domain_run(hz_domain* d, int num_hori, double dist_search, int refine_fac
         const double cyc = (double)c.cyc_refill + (double)c.cyc_trav;
         stats->refill_frac = cyc > 0 ? (double)c.cyc_refill / cyc : 0.0;
         stats->lanes_per_iter = c.trav_iters ? (double)c.nodes / (double)c.trav_iters : 0.0;
-        // cells whose walk was cast: the listed near cells and the border cells
-        stats->cert_walk_cells = cert_run ? (double)ts_host.nfix + (double)ranges_host[3] : 0.0;
+        // cells that ran the walk: the border cells and the other near cells
+        stats->cert_walk_cells = cert_run ? (double)ranges_host[3] + (double)ranges_host[5] : 0.0;
         stats->t_certify_ms = ms_cert;
         stats->t_cert_walk_ms = ms_cwalk;
         stats->t_border_walk_ms = ms_bwalk;
+        stats->t_cert_lattice_ms = ms_lattice;
         stats->spec_rays = (double)c.spec_rays;
         stats->t_total_ms = t1 - t0;
         stats->lon0 = d->lon0;

@@ -126,72 +126,25 @@ struct WalkParams {
 constexpr int kMaxLattice = 180;  // 90 deg
 
 // ---------------------------------------------------------------------------
-// Walk equivalence of the lattice searches (certify.hip, trace.hip MODE 3)
-//
-// The lattice searches (climb, learned start) return, per azimuth, the top
-// hit index of the 0.5 deg lattice; the reference walk (:521-546) returns
-// the same value whenever the hits it explores are monotone in elevation.
-// Hits can be non-monotone only where a ray passes below a point at which
-// the mesh surface has an edge with one triangle (the domain border, the
-// holes grid_type 1 leaves at vertices with fewer than six cells, :299): a
-// lower ray can slip through such a gap while a higher ray hits the terrain
-// beyond it.  For each (cell, azimuth) the certificate kb is the lowest
-// lattice index whose ray passes above every such boundary point within
-// reach of that azimuth's vertical plane; rays at or above kb see monotone
-// hits.  Walking the azimuths in the walk's order with its carried lattice
-// index k (known), an azimuth is re-cast with walk semantics when the walk
-// could explore a ray below kb, and otherwise takes the lattice result.
-// kb = kCertNone: no boundary point in reach (any ray is safe);
-// kb = kCertAll:  cast whenever the start is known.
+// Where the lattice searches equal the reference walk (certify.hip): cells
+// with no mesh-boundary point within reach of their rays see hits that are
+// monotone in elevation, so the lattice search returns the walk's values
+// there; the certified searches run the walk on every other cell.
 // ---------------------------------------------------------------------------
-constexpr int kCertNone = -999;
-constexpr int kCertAll = 999;
-constexpr int kWalkFloor = -170;  // the walk's -85 deg floor in lattice steps (:537)
 
-// Does the walk have to be cast for this azimuth?  known / k: the walk's
-// lattice index at the start of the azimuth; top: lattice top hit index
-// (-1: index 0 misses); kb: certified bound after the backward pass.
-__host__ __device__ inline bool cert_cast(bool known, int k, int top, int kb) {
-    if (!known) return false;  // the backward pass left only safe azimuths here
-    if (k < kb) return true;
-    return top >= 0 ? top < kb : kb > 0;
-}
-
-// Walk state after an azimuth taken from the lattice result: from below
-// (k <= top) the walk ends on the first miss top + 1, from above on the top
-// hit; with no hit at index 0 it descends to an unknown negative index.
-__host__ __device__ inline void cert_skip_state(bool& known, int& k, int top) {
-    if (top >= 0) {
-        k = (known && k > top) ? top : top + 1;
-        known = true;
-    } else {
-        known = false;
-    }
-}
-
-// Lattice top index of a clipped lattice result h [rad] (0 or (2 top + 1) a).
-__host__ __device__ inline int lattice_top(double h, double hori_acc) {
-    if (!(h > 0.0)) return -1;
-    return (int)rint(h / (2.0 * hori_acc) - 0.5);
-}
-
-// Mesh-boundary points of a domain in a uniform x/y grid (CSR), for the
-// certificate.  Built once per domain and search distance.
+// Mesh-boundary points of a domain in a uniform x/y grid (CSR).  Built once
+// per domain and search distance.
 struct CertIndex {
     float4* pts = nullptr;           // boundary points (FP32 ENU x, y, z), bucket order
     int* off = nullptr;              // nx * ny + 1 bucket offsets
     unsigned char* near = nullptr;   // per bucket: a boundary point may be in reach
-    double2* az = nullptr;           // cos / sin of the L azimuths (host glibc)
-    double* tanv = nullptr;          // tan(k * 0.5 deg), k = kWalkFloor .. kMaxLattice - 1
     int npts = 0, nx = 0, ny = 0, rb = 0;  // rb: bucket radius scanned around a cell
     double x0 = 0.0, y0 = 0.0, S = 1.0;    // grid origin and bucket size [m]
-    double lw = 0.0;                 // lateral band: longest boundary edge (+ margin) [m]
+    double lw = 0.0;                 // longest boundary edge (+ margin) [m]
     double reach = 0.0;              // (double)(float)dist_search [m]
-    double r2max = 0.0;              // squared horizontal reach of a band point
-    double shift = 0.0;              // azimuth of the first direction [rad]
-    int L = 0;                       // azimuths the az table holds
+    double r2max = 0.0;              // squared horizontal distance of a boundary point in
+                                     // reach: (reach + lw)^2 + lw^2
     float dist_key = -1.0f;          // dist_search the index was built for
-    int az_key_nhori = 0, az_key_rf = 0;
 };
 
 // Boundary points (cells next to a vertex with fewer than six cells, i.e.
@@ -200,25 +153,8 @@ struct CertIndex {
 // cell elevations (bounds the vertical offset of a boundary point).
 int cert_index_build(CertIndex& ci, const CellFrame* cells, const float* vert, int ncell,
                      const int* tri, long long ntri, const float bounds_xy[4], double hmin,
-                     double hmax, float tfar, int num_hori, int refine_factor, hipStream_t st,
-                     std::string& err);
+                     double hmax, float tfar, hipStream_t st, std::string& err);
 void cert_index_free(CertIndex& ci);
-// Host tables of the certificate (shared with the CPU restatement's rules):
-// azimuth cos / sin and the lattice tangents.
-void cert_tables(int num_hori, int refine_factor, double2* az, double* tanv,
-                 double* shift_out);
-constexpr int kCertTan = kMaxLattice - kWalkFloor;  // entries of tanv
-// Lattice index bound of a tangent bound B (max of w / u, rounded up to
-// float): the lowest index whose tangent exceeds B by the margin.
-constexpr double kCertMargin = 2e-5;
-
-// Certificate of every cell (after the lattice item kernel): per item bound
-// kb[j * ncell + p] (positions p of the trace order), the cells that need
-// the walk appended to fixlist (count in state->nfix), and the sector means
-// and SVF of every other cell written from hitem.
-struct TraceState;
-// skip_border: cells next to a mesh hole are walked separately (neither
-// listed nor written here).
 // Flag bit 1 of the cells with a mesh-boundary point in reach (bit 0 cells
 // excluded), or clear it (clear_only); counts_ranges (10 ints) receives two
 // class counts and the {first, count} position ranges of the three classes
@@ -226,9 +162,6 @@ struct TraceState;
 // with bit 1, the rest) and of the last two together.
 void launch_cert_classes(const CertIndex& ci, CellFrame* cells, int ncell, bool clear_only,
                          int* counts_ranges, hipStream_t st);
-void launch_certify(const CertIndex& ci, const CellFrame* cells, const int* order,
-                    const double* hitem, const WalkParams& p, double* horizon, double* skyview,
-                    int* kb, int* fixlist, TraceState* ts, bool skip_border, hipStream_t st);
 
 // cos / sin of k * (2 * hori_acc), k = 0 .. kMaxLattice, through glibc
 // sincos, and sin((2k + 1) * hori_acc) through glibc sin (host)
@@ -403,8 +336,7 @@ struct TraceCounters {
 constexpr int kQueueStride = 32;
 struct TraceState {
     TraceCounters cnt;
-    int nfix;  // cells the certificate hands to the walk (search modes 1 and 2)
-    int pad[15];
+    int pad[16];
     int queue[8 * kQueueStride];
 };
 static_assert(sizeof(TraceCounters) + 16 * sizeof(int) == 128, "queue offset");
@@ -448,45 +380,22 @@ int launch_trace(const DeviceBvh& bvh, const CellFrame* cells, const int* order,
                  bool counters, double* horizon, double* skyview, TraceState* state,
                  double* spill, double* hitem, bool items_finish, hipStream_t stream,
                  std::string& err);
-// The walk (MODE 3) of the cells fixlist[0 .. state->nfix) lists (trace
-// positions): with kb, the certified walk of the cells k_certify listed
-// (steps of covered azimuths replayed uncast); with kb = hitem = NULL, the
-// plain reference walk.  max_work bounds the list length (sizes the grid);
-// the spill must hold trace_spill_doubles(max_work) doubles.
-int launch_trace_fix(const DeviceBvh& bvh, const CellFrame* cells, const int* order,
-                     const WalkParams& p, int block_size, int refill, bool counters,
-                     double* horizon, double* skyview, TraceState* state, double* spill,
-                     double* hitem, const int* kb, const int* fixlist, int max_work,
-                     hipStream_t stream, std::string& err);
-// The reference walk of the cells list[0 .. state->nfix) names (trace
-// positions; at most max_cells) or of the positions p.range names, each by a
-// team of `team` lanes (2..64, a
-// power of two) that casts the walk's next steps together: the first round
-// of an azimuth casts its first query with up- and down-steps beside it,
-// later rounds the next `team` steps of the run, and the walk takes the
-// first answer that ends its run.  Rays, elevations and results equal the
-// one-lane walk's bit for bit; state->cnt.rays counts the walk's own
-// queries, cnt.spec_rays the ones cast ahead and not used.  With kb and
-// hitem (certified lattice searches) an azimuth whose lattice top and start
-// lie at or above the bound is replayed without casting, as in MODE 3.
-// The teams take cells from one counter of the state (its work queues are
-// cleared first); list = NULL: the positions p.range names; at most
-// grid_blocks workgroups of 128 lanes.
+// The reference walk of the cells at the positions p.range names (at most
+// max_cells), each by a team of `team` lanes (2..64, a power of two) that
+// casts the walk's next steps together: the first round of an azimuth casts
+// its first query with up- and down-steps beside it, later rounds the next
+// `team` steps of the run, and the walk takes the first answer that ends its
+// run.  Rays, elevations and results equal the one-lane walk's bit for bit;
+// state->cnt.rays counts the walk's own queries, cnt.spec_rays the ones cast
+// ahead and not used.  The teams take cells from one counter of the state
+// (its work queues are cleared first); at most grid_blocks workgroups of 128
+// lanes.  Binary trees only.
 int launch_walk_team(const DeviceBvh& bvh, const CellFrame* cells, const int* order,
                      const WalkParams& p, int team, bool counters, double* horizon,
-                     double* skyview, TraceState* state, const double* hitem, const int* kb,
-                     const int* list, int max_cells, int grid_blocks, hipStream_t stream,
-                     std::string& err);
-// Sector means and SVF of the item results hitem at the positions p.range
-// names (all cells without it).
-void launch_items_finish(const double* hitem, const int* order, const WalkParams& p,
-                         double* horizon, double* skyview, hipStream_t stream);
-// Zero the counters, list count and work queues of a trace state.
+                     double* skyview, TraceState* state, int max_cells, int grid_blocks,
+                     hipStream_t stream, std::string& err);
+// Zero the counters and work queues of a trace state.
 void launch_clear_state(TraceState* state, hipStream_t stream);
-// Trace positions of the cells next to a mesh hole (CellFrame flag bit 0),
-// appended to list; count in state->nfix (the state is cleared first).
-void launch_border_list(const CellFrame* cells, const int* order, int ncell, int* list,
-                        TraceState* state, hipStream_t stream);
 size_t trace_spill_doubles(int nwork);
 constexpr int S_FIELDS_HOST = 13;  // 8-byte fields of the per-lane search spill
 

@@ -452,6 +452,38 @@ __device__ __forceinline__ int trav_step4(const Bvh4Node* __restrict__ nodes,
     return 0;
 }
 
+// Whole any-hit query, first against the triangle (leaf-order index hint,
+// -1: none) that occluded this lane's last hit; hint is set to the
+// occluding triangle of a hit.
+template <bool COUNT, int BLOCK>
+__device__ __forceinline__ bool occluded_hint(const BvhNode* __restrict__ nodes,
+                                              const float4* __restrict__ tris, int* stk,
+                                              const Ray& r, int& hint, unsigned& nvisit,
+                                              unsigned& ntest, int& err) {
+    if (hint >= 0) {
+        const float4* t = tris + 3 * (size_t)hint;
+        const float4 t0 = t[0], t1 = t[1], t2 = t[2];
+        if (COUNT) ntest++;
+        if (tri_hit(r, t0.x, t0.y, t0.z, t0.w, t1.x, t1.y, t1.z, t1.w, t2.x)) return true;
+    }
+    int ovf[kStackMax - kLdsStack];
+    int node = kWideBit;  // the root is read in full (128 bytes)
+    int sp = 0;
+    for (int guard = 0;; guard++) {
+        if (guard > (1 << 28)) {
+            err |= 4;
+            return true;
+        }
+        int hit_leaf = 0;
+        int s = trav_step<COUNT, BLOCK, true>(nodes, tris, stk, ovf, r, node, sp, nvisit, ntest,
+                                              err, hit_leaf);
+        if (s) {
+            if (s == 1) hint = hit_leaf;
+            return s == 1;
+        }
+    }
+}
+
 // Whole any-hit query.
 template <bool COUNT, int BLOCK>
 __device__ __forceinline__ bool occluded(const BvhNode* __restrict__ nodes,
@@ -636,8 +668,7 @@ __global__ void k_clear_queue(TraceState* ts) {
 // needed only when a query ends), so that the traversal loop runs with the
 // registers of the traversal alone.  Field f of lane g is spill[f * lanes + g].
 // Fields S_I0.. hold two ints each: {cell, phase}, {azimuth j, lattice index},
-// {sector step, sector index}, {batch domain, unused} or (MODE 3) {trace
-// position p, walk index known}.
+// {sector step, sector index}, {batch domain, unused}.
 enum {
     S_DIR = 0, S_AX = 3, S_E = 6, S_SECTOR = 7, S_SVF = 8,
     S_I0 = 9, S_I1 = 10, S_I2 = 11, S_I3 = 12, S_FIELDS = 13
@@ -653,14 +684,8 @@ static_assert(S_FIELDS == S_FIELDS_HOST, "spill layout");
 // azimuth) pairs instead of cells, since these searches start every azimuth
 // afresh; item results go to hitem and k_items_finish forms the sector means
 // and the SVF.
-// MODE 3: the reference walk of the cells the certificate could not vouch
-// for (the positions fixlist[0 .. ts->nfix)), bit for bit: its elevation,
-// rotations and -85 deg floor test are the walk's own, and kk tracks the
-// lattice index of its elevation.  An azimuth whose lattice top hit index
-// (from hitem) and starting index both lie at or above the certified bound
-// kbuf sees monotone hits, so the walk's steps there are replayed without
-// casting; every other azimuth, and every one without a hit at 0 deg (its
-// end, and so the next start, is unknown), is cast.
+// P.range (device {first, count}; NULL: all cells): the positions of the
+// trace order this launch works on (its cells, or the items of its cells).
 template <int MODE, bool COUNT, int BLOCK, int TREE, bool BATCH, bool ITEMS = false>
 __global__ __launch_bounds__(BLOCK, 4) void k_trace_pt(const BvhNode* __restrict__ nodes,
                                                     const float4* __restrict__ tris, int num_tri,
@@ -672,12 +697,8 @@ __global__ __launch_bounds__(BLOCK, 4) void k_trace_pt(const BvhNode* __restrict
                                                     TraceState* __restrict__ ts,
                                                     double* __restrict__ spill,
                                                     const BatchDomain* __restrict__ doms, int ndom,
-                                                    int total, double* __restrict__ hitem,
-                                                    const int* __restrict__ kbuf,
-                                                    const int* __restrict__ fixlist) {
-    static_assert(!ITEMS || (MODE != 0 && MODE != 3 && !BATCH),
-                  "items: lattice searches of one domain");
-    static_assert(MODE != 3 || (!ITEMS && !BATCH), "the certified walk runs whole cells");
+                                                    int total, double* __restrict__ hitem) {
+    static_assert(!ITEMS || (MODE != 0 && !BATCH), "items: lattice searches of one domain");
     __shared__ int stk_all[kLdsStack * BLOCK];
     // the wave's part of the stack: stk[k * BLOCK + lane] (the base is uniform)
     int* stk = stk_all + (__builtin_amdgcn_readfirstlane(threadIdx.x) & ~63);
@@ -686,10 +707,11 @@ __global__ __launch_bounds__(BLOCK, 4) void k_trace_pt(const BvhNode* __restrict
     const unsigned long long lt_mask = (1ull << lane) - 1ull;
     const int L = P.num_hori * P.refine_factor;
     // size of the work index space (the launcher keeps ncell * L below 2^31)
-    // items: positions [ibase, ibase + npos) of the trace order (P.range)
-    const int ibase = (ITEMS && P.range) ? P.range[0] : 0;
-    const int npos = (ITEMS && P.range) ? P.range[1] : P.ncell;
-    const int N = BATCH ? total : (ITEMS ? npos * L : (MODE == 3 ? ts->nfix : P.ncell));
+    // positions [ibase, ibase + npos) of the trace order (P.range; all cells
+    // without it): items of those cells, or those cells
+    const int ibase = (!BATCH && P.range) ? P.range[0] : 0;
+    const int npos = (!BATCH && P.range) ? P.range[1] : P.ncell;
+    const int N = BATCH ? total : (ITEMS ? npos * L : npos);
     const double a = P.hori_acc;
     int* queue = ts->queue;
     const size_t lanes = (size_t)gridDim.x * BLOCK;
@@ -777,11 +799,8 @@ __global__ __launch_bounds__(BLOCK, 4) void k_trace_pt(const BvhNode* __restrict
                             item_j = rem / nb;
                             item_p = ibase + blk * kItemCells + (rem - item_j * nb);
                             cell = order ? order[item_p] : item_p;
-                        } else if (MODE == 3) {
-                            item_p = fixlist[g];
-                            cell = order ? order[item_p] : item_p;
                         } else {
-                            cell = order ? order[g] : g;
+                            cell = order ? order[ibase + g] : ibase + g;
                         }
                         has_cell = true;
                         hint = -1;
@@ -804,7 +823,6 @@ __global__ __launch_bounds__(BLOCK, 4) void k_trace_pt(const BvhNode* __restrict
                         SI(S_I1) = make_int2(ITEMS ? item_j : 0, 0);
                         SI(S_I2) = make_int2(0, ITEMS ? item_p : 0);
                         if (BATCH) SI(S_I3) = make_int2(dom, 0);
-                        if (MODE == 3) SI(S_I3) = make_int2(item_p, 0);
                     }
                     qb += min(__popcll(need), avail);
                 }
@@ -814,8 +832,6 @@ __global__ __launch_bounds__(BLOCK, 4) void k_trace_pt(const BvhNode* __restrict
                 const int2 i0 = SI(S_I0), i1 = SI(S_I1), i2 = SI(S_I2);
                 const int cell = i0.x;
                 int phase = i0.y, j = i1.x, kk = i1.y, ksec = i2.x, jsec = i2.y;
-                int p3 = 0;
-                if (MODE == 3) p3 = SI(S_I3).x;
                 const CellFrame* lcells = cells;
                 double* lhorizon = horizon;
                 double* lskyview = skyview;
@@ -832,7 +848,7 @@ __global__ __launch_bounds__(BLOCK, 4) void k_trace_pt(const BvhNode* __restrict
                 D3 dir = {SV(S_DIR), SV(S_DIR + 1), SV(S_DIR + 2)};
                 D3 ax = {0.0, 0.0, 0.0};
                 double e = 0.0;
-                if (MODE == 0 || MODE == 3) {
+                if (MODE == 0) {
                     ax = {SV(S_AX), SV(S_AX + 1), SV(S_AX + 2)};
                     e = SV(S_E);
                 }
@@ -856,37 +872,7 @@ __global__ __launch_bounds__(BLOCK, 4) void k_trace_pt(const BvhNode* __restrict
                 while (true) {
                     double h = 0.0;
                     if (phase == P_AZSTART) {
-                        if (MODE == 3) {
-                            // the reference walk, except where the certificate
-                            // shows the hits it would explore to be monotone:
-                            // then its steps are replayed without casting
-                            const size_t at = (size_t)j * P.ncell + p3;
-                            const int kbj = kbuf ? kbuf[at] : kCertAll;
-                            const int top = kbuf ? lattice_top(hitem[at], a) : -1;
-                            ax = unit_cross(dir, n);
-                            if (top >= 0 && kk >= kbj && top >= kbj) {
-                                if (kk <= top) {
-                                    while (kk <= top) {  // hits up to top, miss at top + 1
-                                        e += 2.0 * a;
-                                        dir = rodrigues(dir, ax, P.elev_sin_2ha, P.elev_cos_2ha);
-                                        kk++;
-                                    }
-                                    h = e - a;
-                                } else {
-                                    while (kk > top) {  // misses down to the hit at top
-                                        e -= 2.0 * a;
-                                        dir = rodrigues(dir, ax, -P.elev_sin_2ha, P.elev_cos_2ha);
-                                        kk--;
-                                    }
-                                    h = e + a;
-                                }
-                            } else {
-                                phase = P_FIRST;
-                                q = dir;
-                                issue = true;
-                                break;
-                            }
-                        } else {
+                        {
                             phase = P_FIRST;
                             if (MODE == 0) {
                                 ax = unit_cross(dir, n);
@@ -900,13 +886,12 @@ __global__ __launch_bounds__(BLOCK, 4) void k_trace_pt(const BvhNode* __restrict
                             issue = true;
                             break;
                         }
-                    } else if (MODE == 0 || MODE == 3) {  // reference walk (:521-546)
+                    } else if (MODE == 0) {  // reference walk (:521-546)
                         if (phase == P_FIRST) {
                             if (last_hit) {
                                 phase = P_UP;
                                 e += 2.0 * a;
                                 dir = rodrigues(dir, ax, P.elev_sin_2ha, P.elev_cos_2ha);
-                                if (MODE == 3) kk++;
                                 q = dir;
                                 issue = true;
                                 break;
@@ -915,7 +900,6 @@ __global__ __launch_bounds__(BLOCK, 4) void k_trace_pt(const BvhNode* __restrict
                                 phase = P_DOWN;
                                 e -= 2.0 * a;
                                 dir = rodrigues(dir, ax, -P.elev_sin_2ha, P.elev_cos_2ha);
-                                if (MODE == 3) kk--;
                                 q = dir;
                                 issue = true;
                                 break;
@@ -926,8 +910,7 @@ __global__ __launch_bounds__(BLOCK, 4) void k_trace_pt(const BvhNode* __restrict
                                 if (!(e > kPi)) {
                                     e += 2.0 * a;
                                     dir = rodrigues(dir, ax, P.elev_sin_2ha, P.elev_cos_2ha);
-                                    if (MODE == 3) kk++;
-                                    q = dir;
+                                        q = dir;
                                     issue = true;
                                     break;
                                 }
@@ -938,7 +921,6 @@ __global__ __launch_bounds__(BLOCK, 4) void k_trace_pt(const BvhNode* __restrict
                             if (!last_hit && e > P.elev_thresh) {
                                 e -= 2.0 * a;
                                 dir = rodrigues(dir, ax, -P.elev_sin_2ha, P.elev_cos_2ha);
-                                if (MODE == 3) kk--;
                                 q = dir;
                                 issue = true;
                                 break;
@@ -1024,7 +1006,6 @@ __global__ __launch_bounds__(BLOCK, 4) void k_trace_pt(const BvhNode* __restrict
                     SI(S_I0) = make_int2(cell, phase);
                     SI(S_I1) = make_int2(j, kk);
                     SI(S_I2) = make_int2(ksec, jsec);
-                    if (MODE == 3) SI(S_I3) = make_int2(p3, 0);
                     node = kWideBit;  // the root is read in full
                     sp = 0;
                     steps = 0;
@@ -1034,7 +1015,7 @@ __global__ __launch_bounds__(BLOCK, 4) void k_trace_pt(const BvhNode* __restrict
                     SV(S_DIR + 2) = dir.z;
                     SV(S_SECTOR) = sector;
                     SV(S_SVF) = svf;
-                    if (MODE == 0 || MODE == 3) {
+                    if (MODE == 0) {
                         SV(S_AX) = ax.x;
                         SV(S_AX + 1) = ax.y;
                         SV(S_AX + 2) = ax.z;
@@ -1123,9 +1104,8 @@ template <int TEAM, bool COUNT, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_walk_team(
     const BvhNode* __restrict__ nodes, const float4* __restrict__ tris, int num_tri,
     const CellFrame* __restrict__ cells, const int* __restrict__ order,
-    const int* __restrict__ list, WalkParams P, double* __restrict__ horizon,
-    double* __restrict__ skyview, TraceState* __restrict__ ts, const double* __restrict__ hitem,
-    const int* __restrict__ kbuf) {
+    WalkParams P, double* __restrict__ horizon, double* __restrict__ skyview,
+    TraceState* __restrict__ ts) {
     static_assert(TEAM >= 2 && TEAM <= 64 && (TEAM & (TEAM - 1)) == 0, "team size");
     __shared__ int stk_all[kLdsStack * BLOCK];
     int* stk = stk_all + (__builtin_amdgcn_readfirstlane(threadIdx.x) & ~63);
@@ -1133,20 +1113,19 @@ __global__ __launch_bounds__(BLOCK) void k_walk_team(
     const int t = lane & (TEAM - 1);
     const int tb = lane - t;
     const unsigned long long full = TEAM == 64 ? ~0ull : ((1ull << TEAM) - 1ull);
-    constexpr int U = TEAM / 2 - 1;  // up-steps of round 0 (lanes 1 .. U)
-    constexpr int D = TEAM - 1 - U;  // down-steps of round 0 (lanes U + 1 .. TEAM - 1)
     unsigned long long rays = 0, spec = 0;
     unsigned nvisit = 0, ntest = 0;
     int err = 0;
-    // cells from one counter (queue[0]): the list's entries, or the
-    // positions of P.range; every lane of a team takes the same cell
-    const int count = list ? ts->nfix : P.range[1];
+    // cells from one counter (queue[0]) over the positions of P.range;
+    // every lane of a team takes the same cell
+    const int count = P.range[1];
     while (true) {
         int idx = 0;
         if (t == 0) idx = atomicAdd(&ts->queue[0], 1);
         idx = __shfl(idx, tb, 64);
         if (idx >= count) break;
-        const int p = list ? list[idx] : P.range[0] + idx;
+        const int p = P.range[0] + idx;
+        int hint = -1;  // this lane's last occluding triangle in this cell
         const int cell = order ? order[p] : p;
         const CellFrame& f = cells[cell];
         const D3 n = {f.n[0], f.n[1], f.n[2]};
@@ -1157,36 +1136,15 @@ __global__ __launch_bounds__(BLOCK) void k_walk_team(
         const bool have = num_tri > 0;
         D3 dir = rodrigues(north, n, P.shift_sin, P.shift_cos);
         double e = 0.0;
-        int kk = 0;  // lattice index of e (for the certified replay)
         double sector = 0.0, svf = 0.0;
         int ksec = 0, jsec = 0;
         for (int j = 0; j < L; j++) {
             const D3 ax = unit_cross(dir, n);
             double h;
             bool done = false;
-            if (kbuf) {  // certified azimuth: the walk's steps replayed uncast
-                const size_t at = (size_t)j * P.ncell + p;
-                const int kbj = kbuf[at];
-                const int top = lattice_top(hitem[at], a);
-                if (top >= 0 && kk >= kbj && top >= kbj) {
-                    if (kk <= top) {
-                        while (kk <= top) {
-                            e += 2.0 * a;
-                            dir = rodrigues(dir, ax, P.elev_sin_2ha, P.elev_cos_2ha);
-                            kk++;
-                        }
-                        h = e - a;
-                    } else {
-                        while (kk > top) {
-                            e -= 2.0 * a;
-                            dir = rodrigues(dir, ax, -P.elev_sin_2ha, P.elev_cos_2ha);
-                            kk--;
-                        }
-                        h = e + a;
-                    }
-                    done = true;
-                }
-            }
+            // round 0 casts the azimuth's first query with U up-steps and D
+            // down-steps beside it
+            constexpr int U = TEAM / 2 - 1, D = TEAM - 1 - U;
             int sgn = 0;  // the run's direction once known (+1 up, -1 down)
             for (int round = 0; !done; round++) {
                 // this lane's step: round 0 = start / up / down lanes
@@ -1230,7 +1188,8 @@ __global__ __launch_bounds__(BLOCK) void k_walk_team(
                 if (valid && have) {
                     const Ray r = make_ray(o0, o1, o2, (float)d.x, (float)d.y, (float)d.z, P.tfar,
                                            P.tfar_box);
-                    hit = occluded<COUNT, BLOCK>(nodes, tris, stk, r, nvisit, ntest, err);
+                    hit = occluded_hint<COUNT, BLOCK>(nodes, tris, stk, r, hint, nvisit, ntest,
+                                                      err);
                 }
                 const unsigned long long hb = (__ballot(hit) >> tb) & full;
                 const unsigned long long vb = (__ballot(valid) >> tb) & full;
@@ -1242,12 +1201,12 @@ __global__ __launch_bounds__(BLOCK) void k_walk_team(
                     sgn = (hb & 1ull) ? 1 : -1;
                     if (sgn > 0) {
                         R = U;
-                        vm = (vb >> 1) & ((1ull << U) - 1ull);
+                        vm = (vb >> 1) & ((1ull << U) - 1ull);  // U <= 63
                         am = ~(hb >> 1) & vm;  // misses end an up run
                     } else {
                         R = D;
-                        vm = (vb >> (U + 1)) & ((1ull << D) - 1ull);
-                        am = (hb >> (U + 1)) & vm;  // hits end a down run
+                        vm = D ? ((vb >> (U + 1)) & ((1ull << D) - 1ull)) : 0ull;
+                        am = D ? ((hb >> (U + 1)) & vm) : 0ull;  // hits end a down run
                     }
                 } else {
                     R = TEAM;
@@ -1281,11 +1240,9 @@ __global__ __launch_bounds__(BLOCK) void k_walk_team(
                     if (sgn > 0) {
                         e += 2.0 * a;
                         dir = rodrigues(dir, ax, P.elev_sin_2ha, P.elev_cos_2ha);
-                        kk++;
                     } else {
                         e -= 2.0 * a;
                         dir = rodrigues(dir, ax, -P.elev_sin_2ha, P.elev_cos_2ha);
-                        kk--;
                     }
                 }
                 h = sgn > 0 ? e - a : e + a;
@@ -1390,7 +1347,7 @@ void launch_pt(const DeviceBvh& bvh, const CellFrame* cells, const int* order,
                                    items, p.reserve_blocks)),
                                dim3(BLOCK), 0, st, bvh.nodes, bvh.tris, bvh.num_tri, cells, order,
                                p, refill, kChunk, horizon, skyview, ts, spill, nullptr, 0, 0,
-                               hitem, nullptr, nullptr);
+                               hitem);
             if (finish)
                 hipLaunchKernelGGL(k_items_finish, dim3((p.ncell + 255) / 256), dim3(256), 0, st,
                                    hitem, order, p, horizon, skyview);
@@ -1400,21 +1357,7 @@ void launch_pt(const DeviceBvh& bvh, const CellFrame* cells, const int* order,
     hipLaunchKernelGGL((k_trace_pt<MODE, COUNT, BLOCK, TREE, false>),
                        dim3(pt_grid<MODE, COUNT, BLOCK, TREE, false>(p.ncell)), dim3(BLOCK), 0,
                        st, bvh.nodes, bvh.tris, bvh.num_tri, cells, order, p, refill, kChunk,
-                       horizon, skyview, ts, spill, nullptr, 0, 0, nullptr, nullptr, nullptr);
-}
-
-// The walk of the cells the certificate listed (MODE 3), after the lattice
-// item kernel and k_certify on the same stream.
-template <bool COUNT, int BLOCK, int TREE>
-void launch_pt_fix(const DeviceBvh& bvh, const CellFrame* cells, const int* order,
-                   const WalkParams& p, int refill, double* horizon, double* skyview,
-                   TraceState* ts, double* spill, double* hitem, const int* kb,
-                   const int* fixlist, int max_work, hipStream_t st) {
-    hipLaunchKernelGGL(k_clear_queue, dim3(1), dim3(256), 0, st, ts);
-    hipLaunchKernelGGL((k_trace_pt<3, COUNT, BLOCK, TREE, false>),
-                       dim3(pt_grid<3, COUNT, BLOCK, TREE, false>(max_work)), dim3(BLOCK), 0, st,
-                       bvh.nodes, bvh.tris, bvh.num_tri, cells, order, p, refill, kChunk,
-                       horizon, skyview, ts, spill, nullptr, 0, 0, hitem, kb, fixlist);
+                       horizon, skyview, ts, spill, nullptr, 0, 0, nullptr);
 }
 
 template <int MODE, int BLOCK, int TREE>
@@ -1423,7 +1366,7 @@ void launch_pt_batch(const BatchDomain* doms, int ndom, int total, const WalkPar
     hipLaunchKernelGGL((k_trace_pt<MODE, false, BLOCK, TREE, true>),
                        dim3(pt_grid<MODE, false, BLOCK, TREE, true>(total)), dim3(BLOCK), 0, st,
                        nullptr, nullptr, 0, nullptr, nullptr, p, refill, chunk, nullptr, nullptr,
-                       ts, spill, doms, ndom, total, nullptr, nullptr, nullptr);
+                       ts, spill, doms, ndom, total, nullptr);
 }
 
 inline int tree_kind(bool multi, int width) { return width == 4 ? 2 : (multi ? 1 : 0); }
@@ -1470,22 +1413,6 @@ void launch_block(int block, const DeviceBvh& bvh, const CellFrame* cells, const
     else
         launch_one<MODE, COUNT, 128>(bvh, cells, order, p, kernel, multi, refill, horizon,
                                      skyview, ts, spill, hitem, finish, st);
-}
-
-template <bool COUNT, int BLOCK>
-void fix_tree(int tk, const DeviceBvh& bvh, const CellFrame* cells, const int* order,
-              const WalkParams& p, int refill, double* horizon, double* skyview, TraceState* ts,
-              double* spill, double* hitem, const int* kb, const int* fixlist, int max_work,
-              hipStream_t st) {
-    if (tk == 2)
-        launch_pt_fix<COUNT, BLOCK, 2>(bvh, cells, order, p, refill, horizon, skyview, ts, spill,
-                                       hitem, kb, fixlist, max_work, st);
-    else if (tk == 1)
-        launch_pt_fix<COUNT, BLOCK, 1>(bvh, cells, order, p, refill, horizon, skyview, ts, spill,
-                                       hitem, kb, fixlist, max_work, st);
-    else
-        launch_pt_fix<COUNT, BLOCK, 0>(bvh, cells, order, p, refill, horizon, skyview, ts, spill,
-                                       hitem, kb, fixlist, max_work, st);
 }
 
 template <int MODE, int BLOCK>
@@ -1564,37 +1491,28 @@ int launch_trace_batch(const BatchDomain* doms, int ndom, int total, bool multi,
 template <int TEAM, bool COUNT>
 void launch_team(const DeviceBvh& bvh, const CellFrame* cells, const int* order,
                  const WalkParams& p, double* horizon, double* skyview, TraceState* ts,
-                 const double* hitem, const int* kb, const int* list, int max_cells,
-                 int grid_blocks, hipStream_t st) {
+                 int max_cells, int grid_blocks, hipStream_t st) {
     constexpr int B = 128;
     const long long lanes = (long long)max_cells * TEAM;
     const int grid = (int)std::max<long long>(
         1, std::min<long long>(grid_blocks, (lanes + B - 1) / B));
     hipLaunchKernelGGL(k_clear_queue, dim3(1), dim3(256), 0, st, ts);
     hipLaunchKernelGGL((k_walk_team<TEAM, COUNT, B>), dim3(grid), dim3(B), 0, st, bvh.nodes,
-                       bvh.tris, bvh.num_tri, cells, order, list, p, horizon, skyview, ts, hitem,
-                       kb);
+                       bvh.tris, bvh.num_tri, cells, order, p, horizon, skyview, ts);
 }
 
 template <bool COUNT>
 void team_size(int team, const DeviceBvh& bvh, const CellFrame* cells, const int* order,
                const WalkParams& p, double* horizon, double* skyview, TraceState* ts,
-               const double* hitem, const int* kb, const int* list, int max_cells,
-               int grid_blocks, hipStream_t st) {
+               int max_cells, int grid_blocks, hipStream_t st) {
     switch (team) {
-        case 2: launch_team<2, COUNT>(bvh, cells, order, p, horizon, skyview, ts, hitem, kb, list, max_cells, grid_blocks, st); break;
-        case 4: launch_team<4, COUNT>(bvh, cells, order, p, horizon, skyview, ts, hitem, kb, list, max_cells, grid_blocks, st); break;
-        case 8: launch_team<8, COUNT>(bvh, cells, order, p, horizon, skyview, ts, hitem, kb, list, max_cells, grid_blocks, st); break;
-        case 32: launch_team<32, COUNT>(bvh, cells, order, p, horizon, skyview, ts, hitem, kb, list, max_cells, grid_blocks, st); break;
-        case 64: launch_team<64, COUNT>(bvh, cells, order, p, horizon, skyview, ts, hitem, kb, list, max_cells, grid_blocks, st); break;
-        default: launch_team<16, COUNT>(bvh, cells, order, p, horizon, skyview, ts, hitem, kb, list, max_cells, grid_blocks, st); break;
+        case 2: launch_team<2, COUNT>(bvh, cells, order, p, horizon, skyview, ts, max_cells, grid_blocks, st); break;
+        case 4: launch_team<4, COUNT>(bvh, cells, order, p, horizon, skyview, ts, max_cells, grid_blocks, st); break;
+        case 8: launch_team<8, COUNT>(bvh, cells, order, p, horizon, skyview, ts, max_cells, grid_blocks, st); break;
+        case 32: launch_team<32, COUNT>(bvh, cells, order, p, horizon, skyview, ts, max_cells, grid_blocks, st); break;
+        case 64: launch_team<64, COUNT>(bvh, cells, order, p, horizon, skyview, ts, max_cells, grid_blocks, st); break;
+        default: launch_team<16, COUNT>(bvh, cells, order, p, horizon, skyview, ts, max_cells, grid_blocks, st); break;
     }
-}
-
-void launch_items_finish(const double* hitem, const int* order, const WalkParams& p,
-                         double* horizon, double* skyview, hipStream_t st) {
-    hipLaunchKernelGGL(k_items_finish, dim3((p.ncell + 255) / 256), dim3(256), 0, st, hitem, order,
-                       p, horizon, skyview);
 }
 
 void launch_clear_state(TraceState* ts, hipStream_t st) {
@@ -1603,40 +1521,19 @@ void launch_clear_state(TraceState* ts, hipStream_t st) {
 
 int launch_walk_team(const DeviceBvh& bvh, const CellFrame* cells, const int* order,
                      const WalkParams& p, int team, bool counters, double* horizon,
-                     double* skyview, TraceState* ts, const double* hitem, const int* kb,
-                     const int* list, int max_cells, int grid_blocks, hipStream_t st,
-                     std::string& err) {
+                     double* skyview, TraceState* ts, int max_cells, int grid_blocks,
+                     hipStream_t st, std::string& err) {
     if (bvh.width != 2) {
         err = "the cooperative walk traverses binary trees only";
         return -1;
     }
     max_cells = std::max(1, std::min(max_cells, p.ncell));
     if (counters)
-        team_size<true>(team, bvh, cells, order, p, horizon, skyview, ts, hitem, kb, list,
-                        max_cells, grid_blocks, st);
+        team_size<true>(team, bvh, cells, order, p, horizon, skyview, ts, max_cells,
+                        grid_blocks, st);
     else
-        team_size<false>(team, bvh, cells, order, p, horizon, skyview, ts, hitem, kb, list,
-                         max_cells, grid_blocks, st);
-    HZ_HIP(hipGetLastError(), err);
-    return 0;
-}
-
-int launch_trace_fix(const DeviceBvh& bvh, const CellFrame* cells, const int* order,
-                     const WalkParams& p, int block, int refill, bool counters, double* horizon,
-                     double* skyview, TraceState* ts, double* spill, double* hitem, const int* kb,
-                     const int* fixlist, int max_work, hipStream_t st, std::string& err) {
-    if (refill < 1 || refill > 64) refill = 40;
-    const int tk = tree_kind(bvh.leaf_size > 1, bvh.width);
-    max_work = std::max(1, std::min(max_work, p.ncell));
-#define HZ_FIX(C, B) \
-    fix_tree<C, B>(tk, bvh, cells, order, p, refill, horizon, skyview, ts, spill, hitem, kb, \
-                   fixlist, max_work, st)
-    if (counters) {
-        if (block == 64) HZ_FIX(true, 64); else if (block == 256) HZ_FIX(true, 256); else HZ_FIX(true, 128);
-    } else {
-        if (block == 64) HZ_FIX(false, 64); else if (block == 256) HZ_FIX(false, 256); else HZ_FIX(false, 128);
-    }
-#undef HZ_FIX
+        team_size<false>(team, bvh, cells, order, p, horizon, skyview, ts, max_cells,
+                         grid_blocks, st);
     HZ_HIP(hipGetLastError(), err);
     return 0;
 }

@@ -63,13 +63,15 @@ typedef struct hz_options {
                              persistent kernel: 0 (default) = one work item per (cell,
                              azimuth), 1 = one per cell (as the walk); same results */
     int certify;          /* lattice searches (HZ_MODE_CLIMB, HZ_MODE_LEARNED): 1 (default) =
-                             equal to the reference walk: a geometric certificate shows, per
-                             cell and azimuth, where hits are monotone in elevation over what
-                             the walk explores (no mesh-boundary point below those rays), and
-                             the walk's own steps are cast wherever it does not; 0 = the plain
-                             lattice search, which differs from the walk where its rays pass
-                             through mesh holes.  Needs (cell, azimuth) work items (persistent
-                             kernel, whole_cells 0); without memory for them the walk runs */
+                             equal to the reference walk: cells with a mesh-boundary point
+                             (domain border, holes at vertices with fewer than six cells)
+                             within reach run the reference walk itself, bit for bit; every
+                             other cell sees hits monotone in elevation, where the lattice
+                             search returns the walk's values (to the walk's FP64 rounding of
+                             its elevation, ~1e-14 deg); 0 = the plain lattice search, which
+                             differs from the walk where its rays pass through mesh holes.
+                             Needs (cell, azimuth) work items (persistent kernel, whole_cells
+                             0); without memory for them the walk runs everywhere */
     int reserved[2];
 } hz_options;
 
@@ -95,15 +97,17 @@ typedef struct hz_stats {
                                     time spent refilling lanes (search state, next query) */
     double lanes_per_iter;       /* collect_counters, persistent kernel: live lanes (of 64) per
                                     traversal iteration */
-    double cert_walk_cells;      /* certified lattice searches: cells whose walk was cast in
-                                    part (the rest took the lattice results unchanged) */
-    double t_certify_ms;         /* certified lattice searches: certificate kernel (HIP events) */
-    double t_cert_walk_ms;       /* certified lattice searches: the walk of the listed cells
-                                    (from the certificate to the end of the search) */
-    double t_border_walk_ms;     /* certified lattice searches: the walk of the cells next to
-                                    mesh holes, run from the start beside the lattice items */
-    double spec_rays;            /* queries the cooperative walk cast ahead and did not use
-                                    (not in num_rays) */
+    double cert_walk_cells;      /* certified lattice searches: cells that ran the reference
+                                    walk (a mesh-boundary point in reach); the others ran the
+                                    lattice search */
+    double t_certify_ms;         /* certified lattice searches: classification of the cells
+                                    (before the BVH build; HIP events) */
+    double t_cert_walk_ms;       /* certified lattice searches: the walk of the near cells */
+    double t_border_walk_ms;     /* certified lattice searches: the team walk of the cells next
+                                    to mesh holes (beside the other two launches) */
+    double spec_rays;            /* queries the team walk cast ahead and did not use (not in
+                                    num_rays) */
+    double t_cert_lattice_ms;    /* certified lattice searches: the far cells' lattice search */
 } hz_stats;
 
 /*

@@ -649,31 +649,25 @@ long long walk_cell(const Setup& S, int i, int mode, bool brute, double* horizon
 
 // ---------------------------------------------------------------------------
 // Certified lattice search (restates the library's certify.hip and the
-// certified walk of trace.hip; no reference counterpart: it reproduces the
-// reference walk, :521-546, from per-azimuth lattice results)
+// certified run of capi.cpp; no reference counterpart: it reproduces the
+// reference walk, :521-546, with the lattice search where that is exact).
+// Cells next to a mesh hole (a corner of an ICON vertex with fewer than six
+// cells) and cells with such a cell within reach run the reference walk;
+// every other cell sees hits monotone in elevation and runs the lattice
+// search, whose results then equal the walk's.
 // ---------------------------------------------------------------------------
-
-const int kCertNone = -999, kCertAll = 999, kWalkFloor = -170, kMaxLat = 180;
-const double kCertMargin = 2e-5;
 
 struct CertSetup {
     std::vector<float> px, py, pz;  // boundary points (FP32 scene vertices)
-    std::vector<int> pcell;         // their cell index
+    std::vector<char> flag;         // per cell: next to a mesh hole
     double lw = 0.0, reach = 0.0, r2max = 0.0;
-    std::vector<double> azc, azs;   // cos / sin of the L azimuths
-    std::vector<double> tanv;       // tan(k * 0.5 deg), k = kWalkFloor .. kMaxLat - 1
-    std::vector<int> ea, eb;        // boundary edges (mesh vertex indices)
-    std::vector<char> flag;         // per cell: next to a mesh hole (walked outright)
-    double elen = 0.0;              // longest boundary edge
 };
-int g_cert_variant = 1;
 
-// Boundary points: cells that are a corner of an ICON vertex with fewer than
-// six cells; lw: longest triangle edge between two of them (FP32) with the
-// library's margins; tables with the library's expressions.
+// Boundary points; lw: longest triangle edge between two of them (FP32)
+// with the library's margins; r2max: squared horizontal reach.
 void cert_setup(const Setup& S, const int* cov, int nvert, CertSetup& C) {
     const int ncell = S.ncell;
-    std::vector<char> flag(ncell, 0);
+    C.flag.assign(ncell, 0);
     for (int v = 0; v < nvert; v++) {
         int n = 0;
         for (int j = 0; j < 6; j++)
@@ -681,23 +675,21 @@ void cert_setup(const Setup& S, const int* cov, int nvert, CertSetup& C) {
         if (n == 6) continue;
         for (int j = 0; j < 6; j++) {
             int c = cov[(size_t)nvert * j + v];
-            if (c >= 0 && c < ncell) flag[c] = 1;
+            if (c >= 0 && c < ncell) C.flag[c] = 1;
         }
     }
-    C.flag = flag;
     const std::vector<float>& vx = S.scene.vx;
     for (int i = 0; i < ncell; i++)
-        if (flag[i]) {
+        if (C.flag[i]) {
             C.px.push_back(vx[3 * i]);
             C.py.push_back(vx[3 * i + 1]);
             C.pz.push_back(vx[3 * i + 2]);
-            C.pcell.push_back(i);
         }
     float best = 0.0f;
     for (int t = 0; t < S.scene.ntri; t++)
         for (int e = 0; e < 3; e++) {
             int a = S.scene.tri[3 * t + e], b = S.scene.tri[3 * t + (e + 1) % 3];
-            if (a >= ncell || b >= ncell || !flag[a] || !flag[b]) continue;
+            if (a >= ncell || b >= ncell || !C.flag[a] || !C.flag[b]) continue;
             float dx = vx[3 * b] - vx[3 * a], dy = vx[3 * b + 1] - vx[3 * a + 1],
                   dz = vx[3 * b + 2] - vx[3 * a + 2];
             best = std::max(best, std::sqrt(dx * dx + dy * dy + dz * dz));
@@ -706,504 +698,24 @@ void cert_setup(const Setup& S, const int* cov, int nvert, CertSetup& C) {
     C.reach = (double)S.tfar;
     const double ulim = C.reach + C.lw;
     C.r2max = ulim * ulim + C.lw * C.lw;
-    const double d2r = kPi / 180.0;
-    double shift = 0.0;
-    if (S.rf != 1)
-        shift = -((360.0 * d2r) / (2.0 * S.nhori)) + ((360.0 * d2r) / (2.0 * (double)S.L));
-    const double dphi = (360.0 * d2r) / (double)S.L;
-    C.azc.resize(S.L);
-    C.azs.resize(S.L);
-    for (int j = 0; j < S.L; j++) sin_cos(shift + (double)j * dphi, &C.azs[j], &C.azc[j]);
-    for (int k = kWalkFloor; k < kMaxLat; k++) C.tanv.push_back(std::tan((double)k * (0.5 * d2r)));
-    // boundary edges: edges between boundary points used by a number of
-    // triangles other than two (every triangle with such an edge has two
-    // boundary corners, so counting over those triangles is complete)
-    std::vector<std::pair<long long, int>> cnt;
-    for (int t = 0; t < S.scene.ntri; t++)
-        for (int e = 0; e < 3; e++) {
-            int p0 = S.scene.tri[3 * t + e], p1 = S.scene.tri[3 * t + (e + 1) % 3];
-            if (p0 >= ncell || p1 >= ncell || !flag[p0] || !flag[p1]) continue;
-            if (p0 > p1) std::swap(p0, p1);
-            cnt.push_back({(long long)p0 * ncell + p1, 1});
-        }
-    std::sort(cnt.begin(), cnt.end());
-    for (size_t q = 0; q < cnt.size();) {
-        size_t r = q;
-        while (r < cnt.size() && cnt[r].first == cnt[q].first) r++;
-        if (r - q != 2) {
-            int p0 = (int)(cnt[q].first / ncell), p1 = (int)(cnt[q].first % ncell);
-            C.ea.push_back(p0);
-            C.eb.push_back(p1);
-            float dx = vx[3 * p1] - vx[3 * p0], dy = vx[3 * p1 + 1] - vx[3 * p0 + 1],
-                  dz = vx[3 * p1 + 2] - vx[3 * p0 + 2];
-            C.elen = std::max(C.elen, (double)std::sqrt(dx * dx + dy * dy + dz * dz));
-        }
-        q = r;
-    }
 }
 
-// Bounds from the exact crossings of the boundary edges with each azimuth's
-// vertical half-plane: the tangent of elevation w / u of every crossing in
-// front of the origin within reach, with an angular margin of 1e-4 rad.
-void cert_bounds_edges(const Setup& S, const CertSetup& C, int i, int* kb) {
+// Does cell i have a boundary point within horizontal distance sqrt(r2max)
+// in its own frame (east = north x normal)?  Brute force over the points.
+bool cert_near(const Setup& S, const CertSetup& C, int i) {
     const Frame& f = S.frame[i];
-    const int L = S.L;
-    std::vector<double> B(L, -INFINITY);
     const double ox = f.org[0], oy = f.org[1], oz = f.org[2];
     const double n0 = f.n.x, n1 = f.n.y, n2 = f.n.z;
     const double N0 = f.north.x, N1 = f.north.y, N2 = f.north.z;
     const double E0 = N1 * n2 - N2 * n1, E1 = N2 * n0 - N0 * n2, E2 = N0 * n1 - N1 * n0;
-    const double rr = C.reach + C.elen + 1.0;
-    const std::vector<float>& vx = S.scene.vx;
-    for (size_t q = 0; q < C.ea.size(); q++) {
-        double uN[2], uE[2], w[2];
-        for (int t = 0; t < 2; t++) {
-            const int v = t ? C.eb[q] : C.ea[q];
-            const double rx = (double)vx[3 * v] - ox, ry = (double)vx[3 * v + 1] - oy,
-                         rz = (double)vx[3 * v + 2] - oz;
-            uN[t] = rx * N0 + ry * N1 + rz * N2;
-            uE[t] = rx * E0 + ry * E1 + rz * E2;
-            w[t] = rx * n0 + ry * n1 + rz * n2;
-        }
-        if (uN[0] * uN[0] + uE[0] * uE[0] > rr * rr && uN[1] * uN[1] + uE[1] * uE[1] > rr * rr)
-            continue;
-        for (int j = 0; j < L; j++) {
-            const double c = C.azc[j], sn = C.azs[j];
-            const double u0 = uN[0] * c + uE[0] * sn, u1 = uN[1] * c + uE[1] * sn;
-            const double l0 = uE[0] * c - uN[0] * sn, l1 = uE[1] * c - uN[1] * sn;
-            if ((l0 > 0.0 && l1 > 0.0) || (l0 < 0.0 && l1 < 0.0)) continue;
-            double tb;
-            if (l0 == l1) {  // the edge lies in the plane
-                if (u0 <= 0.0 && u1 <= 0.0) continue;
-                if (u0 <= 0.0 || u1 <= 0.0) {
-                    tb = INFINITY;
-                } else {
-                    tb = std::max(w[0] / u0, w[1] / u1);
-                }
-            } else {
-                const double s2 = l0 / (l0 - l1);
-                const double uc = u0 + s2 * (u1 - u0);
-                const double wc = w[0] + s2 * (w[1] - w[0]);
-                if (!(uc > 0.0) || uc > C.reach) continue;
-                tb = wc / uc;
-            }
-            if (tb > B[j]) B[j] = tb;
-        }
-    }
-    const double tm = 1e-4;
-    for (int j = 0; j < L; j++) {
-        if (B[j] == -INFINITY) {
-            kb[j] = kCertNone;
-            continue;
-        }
-        if (!(B[j] < INFINITY) || B[j] * tm >= 1.0) {
-            kb[j] = kMaxLat;
-            continue;
-        }
-        const double bm = (B[j] + tm) / (1.0 - B[j] * tm);
-        int k = kWalkFloor;
-        while (k < kMaxLat && !(C.tanv[k - kWalkFloor] > bm)) k++;
-        kb[j] = k;
-    }
-}
-
-inline float round_up_float(double b) {
-    float f = (float)b;
-    if ((double)f < b) f = std::nextafter(f, INFINITY);
-    return f;
-}
-
-// Per-azimuth lattice index bounds of cell i (before the backward pass):
-// the tangent bound over every boundary point in the band of each azimuth.
-void cert_bounds(const Setup& S, const CertSetup& C, int i, int* kb) {
-    const Frame& f = S.frame[i];
-    const int L = S.L;
-    std::vector<float> B(L, -INFINITY);
-    const double ox = f.org[0], oy = f.org[1], oz = f.org[2];
-    const double n0 = f.n.x, n1 = f.n.y, n2 = f.n.z;
-    const double N0 = f.north.x, N1 = f.north.y, N2 = f.north.z;
-    const double E0 = N1 * n2 - N2 * n1, E1 = N2 * n0 - N0 * n2, E2 = N0 * n1 - N1 * n0;
-    const double ulim = C.reach + C.lw;
     for (size_t q = 0; q < C.px.size(); q++) {
         const double rx = (double)C.px[q] - ox, ry = (double)C.py[q] - oy,
                      rz = (double)C.pz[q] - oz;
         const double uN = rx * N0 + ry * N1 + rz * N2;
         const double uE = rx * E0 + ry * E1 + rz * E2;
-        const double r2 = uN * uN + uE * uE;
-        if (r2 > C.r2max) continue;
-        const double w = rx * n0 + ry * n1 + rz * n2;
-        for (int j = 0; j < L; j++) {
-            const double u = uN * C.azc[j] + uE * C.azs[j];
-            const double lat = uE * C.azc[j] - uN * C.azs[j];
-            if (!(u > 0.0) || u > ulim || std::fabs(lat) > C.lw) continue;
-            const float tb = u <= C.lw ? INFINITY : round_up_float(w / u);
-            if (tb > B[j]) B[j] = tb;
-        }
+        if (uN * uN + uE * uE <= C.r2max) return true;
     }
-    for (int j = 0; j < L; j++) {
-        if (B[j] == -INFINITY) {
-            kb[j] = kCertNone;
-        } else if (!(B[j] < INFINITY)) {
-            kb[j] = kMaxLat;
-        } else {
-            const double bm = (double)B[j] + kCertMargin;
-            int k = kWalkFloor;
-            while (k < kMaxLat && !(C.tanv[k - kWalkFloor] > bm)) k++;
-            kb[j] = k;
-        }
-    }
-}
-
-inline bool cert_cast_o(bool known, int k, int top, int kb) {
-    if (!known) return false;
-    if (k < kb) return true;
-    return top >= 0 ? top < kb : kb > 0;
-}
-
-// Certified search of cell i: lattice results of every azimuth (search mode
-// 1 or 2), the bounds and the backward pass, then the walk's carried lattice
-// index followed forward over the lattice results.  If it never meets an
-// azimuth the certificate does not cover, the lattice results stand (equal
-// to the walk's values up to its FP64 rounding); otherwise (*walked = 1)
-// the cell runs the reference walk itself, replaying uncast the steps of
-// azimuths whose lattice top and start lie at or above the bound.  Returns
-// the rays cast.
-// debugging aid: per-cell counts of the certified walk (cast rays, cast
-// azimuths with no lattice hit at 0 deg, with a start below the bound, with
-// a top below the bound)
-struct CertCellStats {
-    int rays = 0, cast_neg = 0, cast_start = 0, cast_top = 0;
-};
-thread_local CertCellStats* g_cert_cell_stats = nullptr;
-
-// Variant 2 of the certified search of cell i: bounds from the exact
-// boundary-edge crossings; in a cell with any bound, azimuths without a
-// lattice hit at 0 deg continue the lattice search downwards to their first
-// hit (or to index -171), so that the walk's state is known everywhere and
-// the walk itself is replayed exactly (elevation, rotations, floor test),
-// casting only where it would explore a ray below the bound.
-long long cert_cell_v2(const Setup& S, const CertSetup& C, int i, int base_mode,
-                       double* horizon, double* skyview, double* hcell, int* walked,
-                       long long* ntests) {
-    const Frame& f = S.frame[i];
-    const int L = S.L;
-    const double a = S.hori_acc;
-    long long rays = 0;
-    std::vector<V3> hdir(L);
-    std::vector<int> top(L), kb(L);
-    const int kVoid = -1000;
-    V3 dir = rodrigues(f.north, f.n, S.sh_s, S.sh_c);
-    auto ldir = [&](const V3& d, int k) {
-        const int m = k < 0 ? -k : k;
-        const double c = S.lat_c[m], sn = k < 0 ? -S.lat_s[m] : S.lat_s[m];
-        return V3{c * d.x + sn * f.n.x, c * d.y + sn * f.n.y, c * d.z + sn * f.n.z};
-    };
-    cert_bounds_edges(S, C, i, kb.data());
-    bool near = false;
-    for (int j = 0; j < L; j++) near |= kb[j] != kCertNone;
-    for (int j = 0; j < L; j++) {
-        hdir[j] = dir;
-        int k = 0;
-        if (base_mode == 2) {
-            float kf = std::floor(S.pred[(size_t)i * 24 + j] * 2.0f + 0.5f);
-            k = !(kf >= 0.0f) ? 0 : (kf > 180.0f ? 180 : (int)kf);
-        }
-        int t = -1;
-        bool hit = cast(S, f, ldir(dir, k), false, ntests);
-        rays++;
-        if (hit) {
-            while (true) {
-                if (k >= 180) {
-                    t = k;
-                    break;
-                }
-                k++;
-                hit = cast(S, f, ldir(dir, k), false, ntests);
-                rays++;
-                if (!hit) {
-                    t = k - 1;
-                    break;
-                }
-            }
-        } else {
-            while (k > 0) {
-                k--;
-                hit = cast(S, f, ldir(dir, k), false, ntests);
-                rays++;
-                if (hit) {
-                    t = k;
-                    break;
-                }
-            }
-            if (t < 0 && near) {  // on below 0 deg: the walk's descent target
-                t = kVoid;
-                while (k > kWalkFloor - 1) {
-                    k--;
-                    hit = cast(S, f, ldir(dir, k), false, ntests);
-                    rays++;
-                    if (hit) {
-                        t = k;
-                        break;
-                    }
-                }
-            }
-        }
-        top[j] = t;
-        dir = rodrigues(dir, f.n, -S.azim_sin, S.azim_cos);
-    }
-    *walked = 0;
-    if (!near) {
-        for (int j = 0; j < L; j++) hcell[j] = top[j] >= 0 ? (double)(2 * top[j] + 1) * a : 0.0;
-    } else {
-        V3 d = rodrigues(f.north, f.n, S.sh_s, S.sh_c);
-        double e = 0.0;
-        int k = 0;
-        for (int j = 0; j < L; j++) {
-            V3 ax = cross3(d, f.n);
-            normalise(ax);
-            const int t = top[j];
-            const int teff = t == kVoid ? kWalkFloor - 2 : t;
-            double h;
-            if (!(k < kb[j] || teff < kb[j])) {
-                if (t != kVoid && k <= t) {
-                    while (k <= t) {
-                        e += 2.0 * a;
-                        d = rodrigues(d, ax, S.es2, S.ec2);
-                        k++;
-                    }
-                    h = e - a;
-                } else {
-                    while (k > teff && e > S.thresh) {
-                        e -= 2.0 * a;
-                        d = rodrigues(d, ax, -S.es2, S.ec2);
-                        k--;
-                    }
-                    h = e + a;
-                }
-            } else {
-                *walked = 1;
-                if (g_cert_cell_stats) {
-                    if (t < 0) g_cert_cell_stats->cast_neg++;
-                    else if (k < kb[j]) g_cert_cell_stats->cast_start++;
-                    else g_cert_cell_stats->cast_top++;
-                }
-                const long long r0 = rays;
-                bool hit = cast(S, f, d, false, ntests);
-                rays++;
-                if (hit) {
-                    while (hit) {
-                        if (e > kPi) break;
-                        e += 2.0 * a;
-                        d = rodrigues(d, ax, S.es2, S.ec2);
-                        k++;
-                        hit = cast(S, f, d, false, ntests);
-                        rays++;
-                    }
-                    h = e - a;
-                } else {
-                    while (!hit && e > S.thresh) {
-                        e -= 2.0 * a;
-                        d = rodrigues(d, ax, -S.es2, S.ec2);
-                        k--;
-                        hit = cast(S, f, d, false, ntests);
-                        rays++;
-                    }
-                    h = e + a;
-                }
-                if (g_cert_cell_stats) g_cert_cell_stats->rays += (int)(rays - r0);
-            }
-            if (h < 0.0) h = 0.0;
-            hcell[j] = h;
-            d = rodrigues(d, f.n, -S.azim_sin, S.azim_cos);
-        }
-    }
-    for (int J = 0; J < S.nhori; J++) {
-        double m = 0.0;
-        for (int q = 0; q < S.rf; q++) m += hcell[J * S.rf + q];
-        horizon[(size_t)J * S.ncell + i] = rad2deg(m) / (double)S.rf;
-    }
-    double svf = 0.0;
-    for (int j = 0; j < L; j++) {
-        double sn = std::sin(hcell[j]);
-        if (S.svf_type == 0)
-            svf += (1.0 - sn);
-        else if (S.svf_type == 1)
-            svf += (1.0 - (sn * sn));
-        else
-            svf += (1.0 - (sn * sn * sn));
-    }
-    skyview[i] = svf / (double)L;
-    return rays;
-}
-
-long long cert_cell(const Setup& S, const CertSetup& C, int i, int base_mode, double* horizon,
-                    double* skyview, double* hcell, int* walked, long long* ntests) {
-    const Frame& f = S.frame[i];
-    const int L = S.L;
-    const double a = S.hori_acc;
-    long long rays = 0;
-    std::vector<V3> hdir(L);
-    std::vector<int> top(L), kb(L);
-    V3 dir = rodrigues(f.north, f.n, S.sh_s, S.sh_c);
-    auto ldir = [&](const V3& d, int k) {
-        const int m = k < 0 ? -k : k;
-        const double c = S.lat_c[m], sn = k < 0 ? -S.lat_s[m] : S.lat_s[m];
-        return V3{c * d.x + sn * f.n.x, c * d.y + sn * f.n.y, c * d.z + sn * f.n.z};
-    };
-    // cells next to a mesh hole run the walk outright (no lattice search)
-    for (int j = 0; j < L && !C.flag[i]; j++) {
-        hdir[j] = dir;
-        int k = 0;
-        if (base_mode == 2) {
-            float kf = std::floor(S.pred[(size_t)i * 24 + j] * 2.0f + 0.5f);
-            k = !(kf >= 0.0f) ? 0 : (kf > 180.0f ? 180 : (int)kf);
-        }
-        int t = -1;
-        bool hit = cast(S, f, ldir(dir, k), false, ntests);
-        rays++;
-        if (hit) {
-            while (true) {
-                if (k >= 180) {
-                    t = k;
-                    break;
-                }
-                k++;
-                hit = cast(S, f, ldir(dir, k), false, ntests);
-                rays++;
-                if (!hit) {
-                    t = k - 1;
-                    break;
-                }
-            }
-        } else {
-            while (k > 0) {
-                k--;
-                hit = cast(S, f, ldir(dir, k), false, ntests);
-                rays++;
-                if (hit) {
-                    t = k;
-                    break;
-                }
-            }
-        }
-        top[j] = t;
-        dir = rodrigues(dir, f.n, -S.azim_sin, S.azim_cos);
-    }
-    cert_bounds(S, C, i, kb.data());
-    std::vector<int> kbm(kb);  // bounds after the backward pass
-    bool nk_next = false;
-    for (int j = L - 1; j >= 0; j--) {
-        const bool none = kbm[j] == kCertNone;
-        bool nk;
-        if (top[j] >= 0) {
-            nk = !none;
-        } else {
-            if (nk_next) kbm[j] = kCertAll;
-            nk = !none || nk_next;
-        }
-        nk_next = nk;
-    }
-    // does the walk's carried index, followed over the lattice results, ever
-    // reach an azimuth the certificate does not cover?  (Cells next to a mesh
-    // hole run the plain walk outright.)
-    bool known = true;
-    int k = 0;
-    *walked = C.flag[i] ? 1 : 0;
-    if (C.flag[i])
-        for (int j = 0; j < L; j++) kb[j] = kCertAll;
-    for (int j = 0; j < L && !*walked; j++) {
-        const int t = top[j];
-        if (cert_cast_o(known, k, t, kbm[j])) {
-            *walked = 1;
-        } else if (t >= 0) {
-            k = (known && k > t) ? t : t + 1;
-            known = true;
-        } else {
-            known = false;
-        }
-    }
-    if (!*walked) {
-        for (int j = 0; j < L; j++) hcell[j] = top[j] >= 0 ? (double)(2 * top[j] + 1) * a : 0.0;
-    } else {
-        // the reference walk (:507-551), with the steps of every azimuth whose
-        // lattice top and start lie at or above the bound replayed uncast
-        V3 d = rodrigues(f.north, f.n, S.sh_s, S.sh_c);
-        double e = 0.0;
-        k = 0;
-        for (int j = 0; j < L; j++) {
-            V3 ax = cross3(d, f.n);
-            normalise(ax);
-            const int t = top[j];
-            double h;
-            if (t >= 0 && k >= kb[j] && t >= kb[j]) {
-                if (k <= t) {
-                    while (k <= t) {
-                        e += 2.0 * a;
-                        d = rodrigues(d, ax, S.es2, S.ec2);
-                        k++;
-                    }
-                    h = e - a;
-                } else {
-                    while (k > t) {
-                        e -= 2.0 * a;
-                        d = rodrigues(d, ax, -S.es2, S.ec2);
-                        k--;
-                    }
-                    h = e + a;
-                }
-            } else {
-                if (g_cert_cell_stats) {
-                    if (t < 0) g_cert_cell_stats->cast_neg++;
-                    else if (k < kb[j]) g_cert_cell_stats->cast_start++;
-                    else g_cert_cell_stats->cast_top++;
-                }
-                const long long r0 = rays;
-                bool hit = cast(S, f, d, false, ntests);
-                rays++;
-                if (hit) {
-                    while (hit) {
-                        if (e > kPi) break;
-                        e += 2.0 * a;
-                        d = rodrigues(d, ax, S.es2, S.ec2);
-                        k++;
-                        hit = cast(S, f, d, false, ntests);
-                        rays++;
-                    }
-                    h = e - a;
-                } else {
-                    while (!hit && e > S.thresh) {
-                        e -= 2.0 * a;
-                        d = rodrigues(d, ax, -S.es2, S.ec2);
-                        k--;
-                        hit = cast(S, f, d, false, ntests);
-                        rays++;
-                    }
-                    h = e + a;
-                }
-                if (g_cert_cell_stats) g_cert_cell_stats->rays += (int)(rays - r0);
-            }
-            if (h < 0.0) h = 0.0;
-            hcell[j] = h;
-            d = rodrigues(d, f.n, -S.azim_sin, S.azim_cos);
-        }
-    }
-    for (int J = 0; J < S.nhori; J++) {
-        double m = 0.0;
-        for (int q = 0; q < S.rf; q++) m += hcell[J * S.rf + q];
-        horizon[(size_t)J * S.ncell + i] = rad2deg(m) / (double)S.rf;
-    }
-    double svf = 0.0;
-    for (int j = 0; j < L; j++) {
-        double sn = std::sin(hcell[j]);
-        if (S.svf_type == 0)
-            svf += (1.0 - sn);
-        else if (S.svf_type == 1)
-            svf += (1.0 - (sn * sn));
-        else
-            svf += (1.0 - (sn * sn * sn));
-    }
-    skyview[i] = svf / (double)L;
-    return rays;
+    return false;
 }
 
 double now_s() {
@@ -1297,9 +809,9 @@ int orc_enu(const double* clon, const double* clat, const double* hsurf, const d
 // Full horizon / SVF computation with the reference signature plus:
 //   mode      0 = reference walk, 1 = clip-aware lattice climb from 0 deg,
 //             2 = lattice search started at pred[cell * 24 + azimuth] [deg],
-//             3 / 4 = modes 1 / 2 certified against the mesh boundary, with
-//             the walk cast where the certificate does not hold (no BVH
-//             choice: brute is ignored)
+//             3 / 4 = modes 1 / 2 certified: the reference walk on the cells
+//             with a mesh-boundary point in reach, the lattice search on the
+//             others (brute is ignored)
 //   brute     1 = test every triangle for every ray (no BVH)
 //   nthreads  worker threads (<= 0: hardware concurrency)
 //   cell_begin/cell_end  restrict the work to a cell range (outputs for other
@@ -1342,11 +854,12 @@ int orc_horizon_svf(const double* clon, const double* clat, const double* hsurf,
             int e = std::min(b + chunk, cell_end);
             for (int i = b; i < e; i++) {
                 if (mode >= 3) {
-                    int w = 0;
-                    r += g_cert_variant == 2
-                             ? cert_cell_v2(S, C, i, mode - 2, horizon, skyview, hc.data(), &w, &nt)
-                             : cert_cell(S, C, i, mode - 2, horizon, skyview, hc.data(), &w, &nt);
-                    walked_cells += w;
+                    // certified: the walk near mesh boundaries, the lattice
+                    // search (mode 1 / 2) elsewhere
+                    const bool w = C.flag[i] || cert_near(S, C, i);
+                    r += walk_cell(S, i, w ? 0 : mode - 2, false, horizon, skyview, hc.data(),
+                                   &nt);
+                    walked_cells += w ? 1 : 0;
                 } else {
                     r += walk_cell(S, i, mode, brute != 0, horizon, skyview, hc.data(), &nt);
                 }
@@ -1377,128 +890,7 @@ int orc_horizon_svf(const double* clon, const double* clat, const double* hsurf,
     return 0;
 }
 
-// Certificate internals of the listed cells (for checking the certificate):
-// per cell and azimuth the lattice top index (climb), the bound before and
-// after the backward pass, and the reference walk's lattice index at the
-// start of each azimuth; walked[c] as cert_cell decides it.
-int orc_cert_debug(const double* clon, const double* clat, const double* hsurf, const double* vlon,
-                   const double* vlat, const int* cov, int ncell, int nvert, int nhori,
-                   int grid_type, double dist_search, double ray_org_elev, int rf,
-                   const int* cells, int ncells_req, int* top_out, int* kb_out, int* kbm_out,
-                   int* kwalk_out, int* walked_out) {
-    Setup S;
-    setup_domain(S, clon, clat, hsurf, vlon, vlat, cov, ncell, nvert, nhori, grid_type,
-                 dist_search, ray_org_elev, rf, 2, true);
-    CertSetup C;
-    cert_setup(S, cov, nvert, C);
-    const int L = S.L;
-    const double a = S.hori_acc;
-    std::vector<double> hh(L), hor((size_t)nhori * ncell), sky(ncell);
-    for (int q = 0; q < ncells_req; q++) {
-        const int i = cells[q];
-        const Frame& f = S.frame[i];
-        long long nt = 0;
-        int w = 0;
-        cert_cell(S, C, i, 1, hor.data(), sky.data(), hh.data(), &w, &nt);
-        walked_out[q] = w;
-        // lattice tops (climb) and the bounds
-        V3 dir = rodrigues(f.north, f.n, S.sh_s, S.sh_c);
-        for (int j = 0; j < L; j++) {
-            int k = 0, t = -1;
-            auto ld = [&](int kk) {
-                return V3{S.lat_c[kk] * dir.x + S.lat_s[kk] * f.n.x,
-                          S.lat_c[kk] * dir.y + S.lat_s[kk] * f.n.y,
-                          S.lat_c[kk] * dir.z + S.lat_s[kk] * f.n.z};
-            };
-            while (k <= 180 && cast(S, f, ld(k), false, &nt)) t = k++;
-            top_out[(size_t)q * L + j] = t;
-            dir = rodrigues(dir, f.n, -S.azim_sin, S.azim_cos);
-        }
-        cert_bounds(S, C, i, kb_out + (size_t)q * L);
-        std::vector<int> kbm(kb_out + (size_t)q * L, kb_out + (size_t)(q + 1) * L);
-        bool nk_next = false;
-        for (int j = L - 1; j >= 0; j--) {
-            const bool none = kbm[j] == kCertNone;
-            bool nk;
-            if (top_out[(size_t)q * L + j] >= 0) {
-                nk = !none;
-            } else {
-                if (nk_next) kbm[j] = kCertAll;
-                nk = !none || nk_next;
-            }
-            nk_next = nk;
-        }
-        for (int j = 0; j < L; j++) kbm_out[(size_t)q * L + j] = kbm[j];
-        // reference walk: lattice index at each azimuth start
-        V3 d = rodrigues(f.north, f.n, S.sh_s, S.sh_c);
-        double e = 0.0;
-        for (int j = 0; j < L; j++) {
-            kwalk_out[(size_t)q * L + j] = (int)std::lround(e / (2.0 * a));
-            V3 ax = cross3(d, f.n);
-            normalise(ax);
-            bool hit = cast(S, f, d, false, &nt);
-            if (hit) {
-                while (hit) {
-                    if (e > kPi) break;
-                    e += 2.0 * a;
-                    d = rodrigues(d, ax, S.es2, S.ec2);
-                    hit = cast(S, f, d, false, &nt);
-                }
-            } else {
-                while (!hit && e > S.thresh) {
-                    e -= 2.0 * a;
-                    d = rodrigues(d, ax, -S.es2, S.ec2);
-                    hit = cast(S, f, d, false, &nt);
-                }
-            }
-            d = rodrigues(d, f.n, -S.azim_sin, S.azim_cos);
-        }
-    }
-    return 0;
-}
 
-// Per-cell statistics of the certified climb (mode 3) over all cells:
-// out[4 * i + 0..3] = cast rays of the walk, cast azimuths without a lattice
-// hit at 0 deg, cast for a start below the bound, cast for a top below it.
-int orc_cert_cell_stats(const double* clon, const double* clat, const double* hsurf,
-                        const double* vlon, const double* vlat, const int* cov, int ncell,
-                        int nvert, int nhori, int grid_type, double dist_search,
-                        double ray_org_elev, int rf, int* out, int nthreads) {
-    Setup S;
-    setup_domain(S, clon, clat, hsurf, vlon, vlat, cov, ncell, nvert, nhori, grid_type,
-                 dist_search, ray_org_elev, rf, 2, true);
-    CertSetup C;
-    cert_setup(S, cov, nvert, C);
-    if (nthreads <= 0) nthreads = (int)std::max(1u, std::thread::hardware_concurrency());
-    std::vector<double> hor((size_t)nhori * ncell), sky(ncell);
-    std::atomic<int> next(0);
-    auto worker = [&]() {
-        std::vector<double> hh(S.L);
-        while (true) {
-            int i = next.fetch_add(1);
-            if (i >= ncell) break;
-            CertCellStats cs;
-            g_cert_cell_stats = &cs;
-            long long nt = 0;
-            int w = 0;
-            if (g_cert_variant == 2)
-                cert_cell_v2(S, C, i, 1, hor.data(), sky.data(), hh.data(), &w, &nt);
-            else
-                cert_cell(S, C, i, 1, hor.data(), sky.data(), hh.data(), &w, &nt);
-            g_cert_cell_stats = nullptr;
-            out[4 * i] = cs.rays;
-            out[4 * i + 1] = cs.cast_neg;
-            out[4 * i + 2] = cs.cast_start;
-            out[4 * i + 3] = cs.cast_top;
-        }
-    };
-    std::vector<std::thread> th;
-    for (int t = 0; t < nthreads; t++) th.emplace_back(worker);
-    for (auto& t : th) t.join();
-    return 0;
-}
-
-void orc_cert_variant(int v) { g_cert_variant = v; }
 
 // Batch of independent rays against a triangle soup (FP32 scene vertices and
 // triangle indices): hit[i] = OR over triangles of the watertight test.

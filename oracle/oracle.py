@@ -104,9 +104,9 @@ def horizon_svf(dom, num_hori=24, grid_type=1, dist_search=40000.0, ray_org_elev
                 cell_begin=0, cell_end=0, time_budget_s=0.0, out=None, pred=None):
     """Oracle horizon [num_hori, ncell] (deg) and skyview [ncell].
     mode 0 = reference walk, 1 = clip-aware lattice climb, 2 = lattice search
-    started at pred [ncell, 24] (deg), 3 / 4 = modes 1 / 2 certified against
-    the mesh boundary with the walk cast where the certificate does not hold
-    (the library's certify=1)."""
+    started at pred [ncell, 24] (deg), 3 / 4 = modes 1 / 2 certified (the
+    library's certify=1): the reference walk on cells with a mesh-boundary
+    point in reach, the lattice search on the others."""
     clon, clat, hsurf, vlon, vlat, cov = _inputs(dom)
     ncell = clon.size
     if out is None:
