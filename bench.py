@@ -7,11 +7,17 @@ dist_search=40 km, ray_org_elev=0.2 m, refine_factor=1, svf_type=2.
 One step = one full device pass of the hot path with the FP32 scene and cell
 frames already resident in HBM: LBVH build, Morton cell ordering, the
 elevation search of every cell and the fused sector mean / SVF, outputs left
-in HBM.  value = cells per second over the timed steps.
+in HBM.  value = cells per second over the timed steps of the plain GPU
+method (the reference walk, search_mode 0).
+
+The same run times the learned form (predictor + certified lattice search,
+results equal to the walk's) and the certified climb (the lattice search
+without the predictor), each against the walk's results, the C-ABI call end
+to end, and the batched 64-domain workload.
 
 Prints ONE JSON line (rank 0).  The host-CPU baseline (the oracle's C++ port
-of the reference walk, all host threads, time-budgeted sample) is timed in
-the same run unless --no-cpu.
+of the reference walk, every host CPU this process may use, time-budgeted
+sample) is timed in the same run unless --no-cpu.
 """
 
 from __future__ import annotations
@@ -51,6 +57,9 @@ def main():
                     help="first run one device-to-device copy of MIB MiB (a plain copy "
                          "kernel of known traffic, for scaling rocprofv3 FETCH_SIZE / "
                          "WRITE_SIZE)")
+    ap.add_argument("--calibrate-gather", type=int, default=0, metavar="MIB",
+                    help="first run one gather of the 64-byte rows of a MIB MiB table in "
+                         "random order, each row once (scattered reads of known traffic)")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--cpu-threads", type=int, default=0)
     args = ap.parse_args()
@@ -66,6 +75,8 @@ def main():
 
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy)
+    if args.calibrate_gather:
+        calibrate_gather(args.calibrate_gather)
     mode = {"walk": 0, "climb": 1}[args.mode]
     dom_in = synthetic.CONFIGS[args.config]()
     nhori, dist, rf, svf = 24, 40000.0, 1, 2
@@ -106,7 +117,8 @@ def main():
     achieved = bytes_launch / (tr * 1e-3) / 1e9
     traffic = profiled_traffic(args)
     out = {
-        "metric": "horizon+SVF cells/s @nhori=24",
+        "metric": "horizon+SVF cells/s @nhori=24, \u22640.1\u00b0 err; rays/s; learned vs plain "
+                  "speedup",
         "value": value,
         "unit": "cells/s",
         "n_gpus": 1,
@@ -134,6 +146,12 @@ def main():
                      "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
                      "traffic": traffic[0] if traffic else None,
                      "traffic_source": traffic[1] if traffic else None,
+                     "frac_measured_traffic": (traffic[0] / (tr * 1e-3) / 1e9 / PEAK_HBM_GBS
+                                               if traffic else None),
+                     "limiter": "instruction issue under lane divergence (no unit "
+                                "saturated: VALU busy with a minority of lanes active, "
+                                "HBM below peak); the HBM roofline is the bound the "
+                                "kernel is priced against",
                      "kernel": "k_trace", "bytes_per_ray": b_ray,
                      "nodes_per_ray": n_node, "tris_per_ray": n_tri},
         "trace_counters": {"refill_frac": cnt["refill_frac"],
@@ -144,6 +162,7 @@ def main():
     if learned_out:
         out["learned"] = learned_out
         out["learned_vs_plain_speedup"] = learned_out["speedup_vs_plain"]
+        out["learned_vs_climb_speedup"] = learned_out["learned_vs_climb_speedup"]
     out["end_to_end"] = end_to_end_leg(dom_in, opts)
     out["end_to_end_device_prologue"] = end_to_end_leg(dom_in, dict(opts, prologue="device"))
     if not args.no_batch and mode == 0:
@@ -180,10 +199,43 @@ def end_to_end_leg(dom_in, opts, reps=3):
             "ms_download": st["t_download_ms"], "ms_total_library": st["t_total_ms"]}
 
 
+def _timed_runs(dom, args, opts):
+    """warmup + args.steps runs of dom with opts; (wall seconds per step,
+    last stats, mean stats of the timed runs)."""
+    import numpy as np
+    for _ in range(max(1, args.warmup)):
+        dom.run(24, 40000.0, 1, 2, **opts)
+    sts = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        sts.append(dom.run(24, 40000.0, 1, 2, **opts))
+    step = (time.perf_counter() - t0) / args.steps
+    mean = {k: float(np.mean([x[k] for x in sts])) for k in sts[0]}
+    return step, sts[-1], mean
+
+
+def _vs_walk(dom, h_walk, s_walk):
+    import numpy as np
+    h, s = dom.fetch()
+    dh = np.abs(h - h_walk)
+    return {"angles_differing_from_walk": int((dh > 1e-9).sum()),
+            "cells_differing_from_walk": int(np.any(dh > 1e-9, axis=0).sum()),
+            "max_abs_dh_deg": float(dh.max()), "max_abs_dsvf": float(np.abs(s - s_walk).max())}
+
+
 def learned_leg(dom, args, h_walk, s_walk, dom_in, plain_step_s):
-    """Learned-bracket form on the same resident domain: each step runs the
-    LBVH build, cell ordering, raster + descriptors + predictor and the
-    lattice search started at the prediction."""
+    """The lattice searches on the same resident domain, against the walk's
+    results of the timed plain run:
+    * learned (search_mode 2, certified): each step runs the LBVH build, the
+      cell classification, cell ordering, raster + descriptors + predictor
+      and the search (near cells walk, far cells start the lattice search at
+      the prediction); its results equal the walk's;
+    * climb (search_mode 1, certified): the same without the predictor (the
+      lattice search climbs from 0 deg), so learned_vs_climb is what the
+      network gains over the lattice search alone;
+    * learned_uncertified: the plain lattice search everywhere, which misses
+      the walk where rays pass through mesh holes (reported, not the form
+      the speedup is quoted on)."""
     import numpy as np
     from horayzon_extpar_amd import learned
 
@@ -193,36 +245,43 @@ def learned_leg(dom, args, h_walk, s_walk, dom_in, plain_step_s):
         return None
     pred = learned.Predictor.from_file(path)
     dom.set_predictor(pred)
-    opts = dict(search_mode=2)
-    for _ in range(max(1, args.warmup)):
-        dom.run(24, 40000.0, 1, 2, **opts)
-    tp, tt = [], []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        st = dom.run(24, 40000.0, 1, 2, **opts)
-        tp.append(st["t_predict_ms"])
-        tt.append(st["t_trace_ms"])
-    t1 = time.perf_counter()
-    h, s = dom.fetch()
+    n = dom_in.ncell
+    step, st, m = _timed_runs(dom, args, dict(search_mode=2))
+    acc = _vs_walk(dom, h_walk, s_walk)
     P = dom.predictions()
-    step = (t1 - t0) / args.steps
-    diff = np.abs(h - h_walk) > 1e-9
-    cells_diff = np.any(diff, axis=0)
-    border = border_affected(dom_in, 24, 40000.0)
     err = np.abs(P - h_walk.T)
+    out = {"value": n / step, "unit": "cells/s", "ms_per_step": step * 1e3,
+           "certified": True, "ms_predict": m["t_predict_ms"], "ms_trace": m["t_trace_ms"],
+           "ms_classify": m["t_certify_ms"], "ms_near_walk": m["t_cert_walk_ms"],
+           "ms_border_team_walk": m["t_border_walk_ms"],
+           "ms_far_lattice": m["t_cert_lattice_ms"],
+           "cells_walked": int(st["cert_walk_cells"]), "rays": st["num_rays"],
+           "rays_cast_ahead_unused": st["spec_rays"],
+           "rays_per_cell_sector": st["num_rays"] / (n * 24),
+           "speedup_vs_plain": plain_step_s / step, **acc,
+           "prediction_mae_deg": float(err.mean()),
+           "weights": os.path.relpath(path, ROOT)}
+    print(f"[bench] learned (certified): {step * 1e3:.2f} ms/step, trace "
+          f"{m['t_trace_ms']:.2f} ms, {acc['angles_differing_from_walk']} angles differ",
+          file=sys.stderr, flush=True)
+    step_u, st_u, m_u = _timed_runs(dom, args, dict(search_mode=2, certify=0))
+    out["learned_uncertified"] = {"ms_per_step": step_u * 1e3, "ms_trace": m_u["t_trace_ms"],
+                                  "rays": st_u["num_rays"],
+                                  "speedup_vs_plain": plain_step_s / step_u,
+                                  **_vs_walk(dom, h_walk, s_walk)}
     dom.set_predictor(None)
-    return {"value": dom_in.ncell / step, "unit": "cells/s", "ms_per_step": step * 1e3,
-            "ms_predict": float(np.mean(tp)), "ms_trace": float(np.mean(tt)),
-            "rays": st["num_rays"], "rays_per_cell_sector": st["num_rays"] / (dom_in.ncell * 24),
-            "speedup_vs_plain": plain_step_s / step,
-            "angles_differing_from_walk": int(diff.sum()),
-            "cells_differing_from_walk": int(cells_diff.sum()),
-            "angles_differing_off_border": int((diff & ~border).sum()),
-            "angles_border_affected": int(border.sum()),
-            "max_abs_dh_deg": float(np.abs(h - h_walk).max()),
-            "max_abs_dsvf": float(np.abs(s - s_walk).max()),
-            "prediction_mae_deg": float(err.mean()),
-            "weights": os.path.relpath(path, ROOT)}
+    step_c, st_c, m_c = _timed_runs(dom, args, dict(search_mode=1))
+    out["climb"] = {"value": n / step_c, "unit": "cells/s", "ms_per_step": step_c * 1e3,
+                    "certified": True, "ms_trace": m_c["t_trace_ms"],
+                    "ms_near_walk": m_c["t_cert_walk_ms"],
+                    "ms_far_lattice": m_c["t_cert_lattice_ms"],
+                    "cells_walked": int(st_c["cert_walk_cells"]), "rays": st_c["num_rays"],
+                    "speedup_vs_plain": plain_step_s / step_c,
+                    **_vs_walk(dom, h_walk, s_walk)}
+    out["learned_vs_climb_speedup"] = step_c / step
+    print(f"[bench] climb (certified): {step_c * 1e3:.2f} ms/step; learned/climb "
+          f"{step_c / step:.3f}x", file=sys.stderr, flush=True)
+    return out
 
 
 def batch_leg(args):
@@ -282,6 +341,22 @@ def calibrate_copy(mib):
     print(f"[bench] calibration copy: {mib} MiB read + {mib} MiB written", file=sys.stderr,
           flush=True)
     del a, b
+
+
+def calibrate_gather(mib):
+    """One index_select of the 64-byte rows of a mib MiB float32 table in a
+    random permutation: reads every row once (plus 8 bytes of index per
+    row) and writes mib MiB."""
+    import torch
+    rows = mib * (1 << 20) // 64
+    table = torch.ones((rows, 16), dtype=torch.float32, device="cuda")
+    idx = torch.randperm(rows, device="cuda")
+    torch.cuda.synchronize()
+    out = table.index_select(0, idx)
+    torch.cuda.synchronize()
+    print(f"[bench] calibration gather: {rows} rows of 64 B in random order ({mib} MiB + "
+          f"{rows * 8 >> 20} MiB of index read, {mib} MiB written)", file=sys.stderr, flush=True)
+    del table, idx, out
 
 
 def profiled_traffic(args):
@@ -344,32 +419,67 @@ def accuracy_leg(dom_in, h, s, h_orc, s_orc, threads, nsample=48):
     cells = np.sort(rng.choice(dom_in.ncell, nsample, replace=False))
     t0 = time.perf_counter()
     th = np.degrees(orc.true_horizon(dom_in, cells, num_hori=24, grid_type=1,
-                                     nthreads=threads or min(os.cpu_count() or 1, 16)))
+                                     nthreads=threads or host_cpus()["threads"]))
     exp = np.where(th > 0, 0.25 + 0.5 * np.floor(th / 0.5), 0.0)
-    clear = (np.abs(th / 0.5 - np.round(th / 0.5)) * 0.5 > 1e-3)
-    clear &= ~border_affected(dom_in, 24, 40000.0)[:, cells].T
+    grazing = np.abs(th / 0.5 - np.round(th / 0.5)) * 0.5 <= 1e-3
+    border = border_affected(dom_in, 24, 40000.0)[:, cells].T & ~grazing
+    clear = ~grazing & ~border
     got = h[:, cells].T
     agree = np.abs(got - exp) < 1e-9
     err = np.abs(got - np.maximum(th, 0.0))[clear]
     out["fp64_brute_force"] = {
         "cells": int(nsample), "clear_angles": int(clear.sum()),
         "clear_angles_agreeing": int(agree[clear].sum()),
-        "grazing_or_border_angles": int((~clear).sum()),
+        "grazing_angles": int(grazing.sum()),
+        "grazing_angles_agreeing": int(agree[grazing].sum()),
+        "border_angles": int(border.sum()),
+        "border_angles_agreeing": int(agree[border].sum()),
         "max_abs_err_vs_true_deg": float(err.max()) if err.size else None,
         "seconds": time.perf_counter() - t0}
     return out
 
 
+def host_cpus():
+    """The host CPUs this process may run on: os.cpu_count(), the affinity
+    mask, the cgroup CPU quota (cpu.max, in CPUs) and the model name; the
+    thread count is the affinity count, capped by the quota where one is
+    set (threads beyond the quota only time-slice the same CPU time)."""
+    info = {"nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0)),
+            "cgroup_quota_cpus": None, "model": None}
+    try:
+        q, per = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if q != "max":
+            info["cgroup_quota_cpus"] = int(q) / int(per)
+    except (OSError, ValueError):
+        pass
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                info["model"] = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    t = info["affinity"]
+    if info["cgroup_quota_cpus"]:
+        t = max(1, min(t, int(info["cgroup_quota_cpus"])))
+    info["threads"] = t
+    return info
+
+
 def cpu_baseline(dom_in, budget_s, threads):
     """The oracle (C++ port of the reference walk, FP32 CPU BVH, std::thread
     pool with dynamic chunks like TBB's auto-partitioning): the whole C2
-    domain on all host threads (time-budgeted), plus one thread on a block of
-    cells at the domain centre.  Returns the JSON object and the oracle's
-    horizon / skyview of the whole-domain run (NaN where not reached)."""
+    domain on every host CPU this process may use (time-budgeted), plus one
+    thread on a block of cells at the domain centre.  The reference's
+    Embree/TBB build is not available on the host (Embree is not
+    installed), so the port stands in for it.  Returns the JSON object and
+    the oracle's horizon / skyview of the whole-domain run (NaN where not
+    reached)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as orc
 
-    nt = threads or min(os.cpu_count() or 1, 16)
+    cpus = host_cpus()
+    nt = threads or cpus["threads"]
     n = dom_in.ncell
     h_orc, s_orc, st = orc.horizon_svf(dom_in, grid_type=1, nthreads=nt, time_budget_s=budget_s)
     cells = st.cells_done
@@ -380,6 +490,9 @@ def cpu_baseline(dom_in, budget_s, threads):
             "sample": f"{cells} of {n} cells (whole domain in index order, {st.t_trace:.1f} s "
                       f"of tracing on {nt} threads); BVH build {st.t_setup:.1f} s excluded",
             "rays_per_s": st.num_rays / st.t_trace,
+            "host": {"nproc": cpus["nproc"], "affinity_cpus": cpus["affinity"],
+                     "cgroup_quota_cpus": cpus["cgroup_quota_cpus"], "cpu_model": cpus["model"],
+                     "embree": "unavailable"},
             "single_thread": {"value": s1.cells_done / s1.t_trace, "unit": "cells/s",
                               "cores": 1, "sample": f"{s1.cells_done} cells at the domain "
                                                     f"centre, {s1.t_trace:.1f} s"}}, h_orc, s_orc

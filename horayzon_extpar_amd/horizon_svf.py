@@ -12,8 +12,11 @@ Differences, all of them supersets:
 * a failing device call raises ``HzError`` (RuntimeError) instead of
   printing and returning garbage;
 * keyword-only ``search_mode``, ``leaf_size``, ``block_size``,
-  ``collect_counters``, ``trace_kernel``, ``refill`` and ``stats`` expose
-  the library options.
+  ``collect_counters``, ``trace_kernel``, ``refill``, ``bvh_morton``,
+  ``bvh_width``, ``prologue``, ``whole_cells``, ``certify`` and ``stats``
+  expose the library options (``hz_options`` in include/horayzon.h).  The
+  default ``search_mode=0`` is the reference walk; the lattice searches
+  (1 climb, 2 learned start) equal it with the default ``certify=1``.
 """
 
 from __future__ import annotations

@@ -3,11 +3,14 @@
 The lattice searches (climb from 0 deg, learned start) find each azimuth's
 top hit on the 0.5 deg lattice; the reference walk (mo_lradtopo_horayzon.cpp
 :521-546) finds the same value wherever the hits it explores are monotone in
-elevation.  With certify=1 (the default) the library bounds, per cell and
-azimuth, the elevation of every mesh-boundary point near the azimuth's
-vertical plane, keeps the lattice result where the walk cannot explore a ray
-below that bound, and runs the reference walk itself (bit for bit, with the
-certified azimuths replayed uncast) in every other cell.
+elevation.  Along a vertical half-plane through a closed terrain surface
+they are, unless the profile crosses a mesh-boundary edge (the domain border
+or the holes grid_type 1 leaves at vertices with fewer than six cells,
+:299).  With certify=1 (the default) the library marks every cell with a
+boundary point within reach of its rays (dist_search plus the longest edge)
+as near; near cells run the reference walk itself, bit for bit, and the far
+cells run the lattice search, whose values then equal the walk's to its FP64
+rounding of the carried elevation.
 
 * The device equals the oracle's certified modes (3: climb, 4: learned
   start) bit for bit: horizons, rays cast and cells walked; both equal the
