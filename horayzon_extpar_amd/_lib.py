@@ -128,6 +128,10 @@ SIGNATURES = [
       ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_longlong)]),
     ("hz_sincos", ctypes.c_int,
      [c_double_p, ctypes.c_longlong, c_double_p, c_double_p, ctypes.c_int]),
+    ("hz_walk_noalloc", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_longlong, ctypes.c_void_p, ctypes.c_longlong, ctypes.c_void_p,
+      ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+      ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
 ]
 
 

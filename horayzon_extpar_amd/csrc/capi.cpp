@@ -86,6 +86,11 @@ int check_run_args(int num_hori, double dist_search, int refine_factor, int svf_
 
 namespace hz {
 int learn_fail(int code, const std::string& msg) { return fail(code, msg); }
+int api_fail(int code, const std::string& msg) { return fail(code, msg); }
+void api_clear_error() { g_err.clear(); }
+int api_check_run_args(int num_hori, double dist_search, int refine_factor, int svf_type) {
+    return check_run_args(num_hori, dist_search, refine_factor, svf_type);
+}
 }  // namespace hz
 
 struct hz_domain {

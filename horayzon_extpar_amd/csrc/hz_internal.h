@@ -292,10 +292,16 @@ struct LbvhWork {
     size_t sort_tmp_bytes = 0;
     void* scan_tmp = nullptr;  // four-wide collapse: node numbering scan
     size_t scan_tmp_bytes = 0;
+    bool borrowed = false;  // buffers laid out in memory the caller owns (lbvh_bind)
 };
 
 int lbvh_reserve(LbvhWork& w, int n, std::string& err);
 void lbvh_free(LbvhWork& w);
+// Lays the buffers lbvh_reserve would allocate for n triangles (or cells)
+// out in the caller's region at base, 256-byte aligned, and returns the
+// bytes they take (base == nullptr: the size only, w untouched).  A bound
+// work area is never freed or grown: builds of more than n fail.
+size_t lbvh_bind(LbvhWork& w, int n, char* base, std::string& err);
 // Builds the BVH of ntri triangles (vertex indices into vert, 3 floats per
 // vertex).  out.nodes / out.tris must hold max(1, ntri - 1) nodes and 3 * ntri
 // float4.  Enqueued on stream; depth is read back only when want_depth.
@@ -397,6 +403,12 @@ int launch_walk_team(const DeviceBvh& bvh, const CellFrame* cells, const int* or
 // Zero the counters and work queues of a trace state.
 void launch_clear_state(TraceState* state, hipStream_t stream);
 size_t trace_spill_doubles(int nwork);
+
+// Error reporting and argument checks of the C-ABI (capi.cpp): api_fail sets
+// the message hz_last_error returns and passes code through.
+int api_fail(int code, const std::string& msg);
+void api_clear_error();
+int api_check_run_args(int num_hori, double dist_search, int refine_factor, int svf_type);
 constexpr int S_FIELDS_HOST = 13;  // 8-byte fields of the per-lane search spill
 
 }  // namespace hz

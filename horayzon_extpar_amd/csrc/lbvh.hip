@@ -670,6 +670,11 @@ inline int grid(int n) { return (n + kBlock - 1) / kBlock; }
 
 int lbvh_reserve(LbvhWork& w, int n, std::string& err) {
     if (n <= w.capacity) return 0;
+    if (w.borrowed) {
+        err = "the build work area holds " + std::to_string(w.capacity) + " triangles, " +
+              std::to_string(n) + " asked";
+        return -1;
+    }
     lbvh_free(w);
     int cap = n;
     size_t nn = (size_t)cap;
@@ -704,7 +709,49 @@ int lbvh_reserve(LbvhWork& w, int n, std::string& err) {
     return 0;
 }
 
+size_t lbvh_bind(LbvhWork& w, int n, char* base, std::string& err) {
+    const size_t nn = (size_t)std::max(n, 1);
+    const size_t nb = (nn + kBlock - 1) / kBlock;
+    int levels = 1;
+    while ((size_t)1 << levels < nb) levels++;
+    // the temporary storage the sort and the scan ask for at this size
+    size_t tmp = 0, stmp = 0;
+    uint64_t* kq = nullptr;
+    int* vq = nullptr;
+    if (rocprim::radix_sort_pairs(nullptr, tmp, kq, kq, vq, vq, (unsigned)nn, 0, 64) !=
+            hipSuccess ||
+        rocprim::exclusive_scan(nullptr, stmp, vq, vq, 0, nn, rocprim::plus<int>()) !=
+            hipSuccess) {
+        err = "rocprim temporary storage query failed";
+        return 0;
+    }
+    const size_t sizes[] = {nn * 8, nn * 8, nn * 4, nn * 4, nn * 2 * sizeof(float4),
+                            nn * 2 * sizeof(float4), nn * 2 * 4, nn * 2 * 4, nn * 2 * 4,
+                            (3 * nn + (size_t)(levels + 1) * nb) * 2 * sizeof(float4),
+                            (nb + 1) * 8 * sizeof(float), 16 * 4, tmp, stmp};
+    void** slots[] = {(void**)&w.keys[0], (void**)&w.keys[1], (void**)&w.vals[0],
+                      (void**)&w.vals[1], (void**)&w.aabb_lo, (void**)&w.aabb_hi,
+                      (void**)&w.child, (void**)&w.range, (void**)&w.parent, &w.boxes,
+                      (void**)&w.part, (void**)&w.bounds, &w.sort_tmp, &w.scan_tmp};
+    size_t off = 0;
+    for (int i = 0; i < 14; i++) {
+        if (base) *slots[i] = base + off;
+        off += (sizes[i] + 255) & ~(size_t)255;
+    }
+    if (base) {
+        w.sort_tmp_bytes = tmp;
+        w.scan_tmp_bytes = stmp;
+        w.capacity = (int)nn;
+        w.borrowed = true;
+    }
+    return off;
+}
+
 void lbvh_free(LbvhWork& w) {
+    if (w.borrowed) {
+        w = LbvhWork();
+        return;
+    }
     void* p[] = {w.keys[0], w.keys[1], w.vals[0], w.vals[1], w.aabb_lo, w.aabb_hi, w.child,
                  w.range, w.parent, w.boxes, w.part, w.bounds, w.sort_tmp, w.scan_tmp};
     for (void* q : p)

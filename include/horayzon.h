@@ -4,8 +4,9 @@
  *
  * Plain C types only (pointers, ints, doubles): a Python ctypes stub, a
  * Cython extern block or a Fortran ISO_C_BINDING interface binds it
- * directly.  All entry points are synchronous: they return once the results
- * are in the caller's arrays (or in device memory, for hz_domain_run).
+ * directly.  All entry points but hz_walk_noalloc are synchronous: they
+ * return once the results are in the caller's arrays (or in device memory,
+ * for hz_domain_run).
  *
  * Error convention: 0 on success, a negative HZ_E* code on failure;
  * hz_last_error() returns the message of the calling thread's last failure.
@@ -306,6 +307,37 @@ int hz_domain_bvh(hz_domain* dom, const hz_options* opt, void* nodes, long long*
  * code on the host (0).  For checking the routine against exact values.
  */
 int hz_sincos(const double* x, long long n, double* s, double* c, int on_device);
+
+/*
+ * The reference walk without device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under
+ * stream capture, tight serving loops, harnesses that forbid hipMalloc).
+ * It computes what hz_domain_run computes with search_mode 0 (the result of
+ * horizon_svf_comp, bit for bit) for a scene the caller already holds in
+ * device memory, in the layout hz_scene_export writes:
+ *   vert32[3 * num_mesh_vertices], tri[3 * num_triangles] (indices into
+ *   vert32, which the call does not check), frame_n / frame_north
+ *   [3 * num_cell] (FP64), org[3 * num_cell] (FP32),
+ * into the caller's device outputs horizon[num_hori * num_cell] (deg, row j =
+ * azimuth j) and skyview[num_cell].  Every intermediate buffer lives in one
+ * static device array of HZ_WALK_NOALLOC_SCRATCH_MIB MiB, which each device
+ * the call runs on holds until the process ends; scenes of more than
+ * HZ_WALK_NOALLOC_MAX_TRIANGLES triangles or HZ_WALK_NOALLOC_MAX_CELLS cells
+ * are refused with HZ_ENOMEM.  The work is enqueued on stream (a
+ * hipStream_t; NULL: the null stream) and the call returns without waiting:
+ * synchronize the stream before reading the outputs.  The first call on a
+ * device also writes a constant table into the array with a synchronous
+ * copy, so make it outside stream capture.  All calls on one device share
+ * the array: calls that overlap on one device race.
+ */
+#define HZ_WALK_NOALLOC_MAX_TRIANGLES 196608
+#define HZ_WALK_NOALLOC_MAX_CELLS 65536
+#define HZ_WALK_NOALLOC_SCRATCH_MIB 96
+int hz_walk_noalloc(const float* vert32, long long num_mesh_vertices, const int* tri,
+                    long long num_triangles, const double* frame_n, const double* frame_north,
+                    const float* org, int num_cell, int num_hori, double dist_search,
+                    int refine_factor, int svf_type, double* horizon, double* skyview,
+                    void* stream);
 
 #ifdef __cplusplus
 }
