@@ -377,34 +377,18 @@ def profiled_traffic(args):
     return None
 
 
-def border_affected(dom_in, num_hori, dist_search, margin_m=3000.0):
-    """[num_hori, ncell] mask of azimuths that can meet the holes of the
-    grid_type-1 mesh at the lattice border (dropped border vertices), where
-    hits need not be monotone in elevation and the reference walk is
-    path-dependent: the cell lies within margin_m of the border, or the
-    azimuth's ground track within dist_search comes that close to it."""
-    import numpy as np
-    hw = dom_in.meta["width_km"] * 500.0 - margin_m
-    hh = dom_in.meta["height_km"] * 500.0 - margin_m
-    phi = np.arange(num_hori) * (2 * np.pi / num_hori)
-    dx, dy = np.sin(phi)[:, None], np.cos(phi)[:, None]
-    x, y = dom_in.cx[None, :], dom_in.cy[None, :]
-    with np.errstate(divide="ignore", invalid="ignore"):
-        tx = np.where(dx > 0, (hw - x) / dx, np.where(dx < 0, (-hw - x) / dx, np.inf))
-        ty = np.where(dy > 0, (hh - y) / dy, np.where(dy < 0, (-hh - y) / dy, np.inf))
-    near = (np.abs(x) > hw) | (np.abs(y) > hh)
-    return (np.minimum(tx, ty) < dist_search * 1.02) | near
-
-
 def accuracy_leg(dom_in, h, s, h_orc, s_orc, threads, nsample=48):
     """The accuracy gate of the run (walk mode): the device horizons against
     the oracle's on every cell the CPU baseline traced (bit-for-bit parity
-    expected), and against the FP64 half-plane true horizon (brute force over
-    all triangles) on nsample seeded cells: on angles that are neither
-    grazing (within 1e-3 deg of a lattice step) nor able to meet the mesh
-    holes at the border, the walk must return the lattice value below the
-    true horizon, 0.25 + 0.5 floor(h / 0.5) deg, or 0."""
+    expected), and against the FP64 half-plane true horizon (every triangle
+    that can meet the azimuth's half-plane) on nsample seeded cells: on angles
+    that are neither grazing (within 1e-3 deg of a lattice step) nor on a
+    search segment passing within two edges (2 km) of a mesh-boundary vertex
+    (where the walk can see through a hole), the walk must return the lattice
+    value below the true horizon, 0.25 + 0.5 floor(h / 0.5) deg, or 0;
+    grazing and border angles are counted apart."""
     import numpy as np
+    from horayzon_extpar_amd import diagnostics
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as orc
 
@@ -422,7 +406,9 @@ def accuracy_leg(dom_in, h, s, h_orc, s_orc, threads, nsample=48):
                                      nthreads=threads or host_cpus()["threads"]))
     exp = np.where(th > 0, 0.25 + 0.5 * np.floor(th / 0.5), 0.0)
     grazing = np.abs(th / 0.5 - np.round(th / 0.5)) * 0.5 <= 1e-3
-    border = border_affected(dom_in, 24, 40000.0)[:, cells].T & ~grazing
+    border = diagnostics.border_azimuths(dom_in.clon, dom_in.clat, dom_in.vlon, dom_in.vlat,
+                                         dom_in.cells_of_vertex, cells, np.arange(24) * 15.0,
+                                         40000.0, 2000.0) & ~grazing
     clear = ~grazing & ~border
     got = h[:, cells].T
     agree = np.abs(got - exp) < 1e-9

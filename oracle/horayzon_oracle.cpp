@@ -964,17 +964,128 @@ int orc_occluded64(const double* vert, int nv, const int* tri, int ntri, const d
 // angle [rad] of the mesh (FP32 scene vertices, evaluated in FP64) within
 // the vertical half-plane of that azimuth and within dist_search of the
 // FP32 ray origin.  Void azimuths return -pi/2.
+//
+// brute != 0 evaluates every triangle at every azimuth.  brute == 0 (the
+// default) visits, per azimuth, only the triangles that can meet its
+// half-plane inside the search ball: a triangle whose bounding box lies
+// farther than dist_search from the origin cannot, and one whose horizontal
+// projection (in the cell's frame) does not contain the origin meets the
+// half-plane of azimuth phi only if phi lies in the arc its corners span as
+// seen from the origin (widened by 1e-9 rad).  Every visited triangle runs
+// the same FP64 arithmetic as in the brute-force form and the maximum does
+// not depend on the order, so both forms return the same bits.
+
+namespace {
+
+// Largest elevation angle of the part of one triangle that lies in the
+// half-plane w = 0, u >= 0 and within |(u, z)| <= sqrt(R2); P[k] = (u, w, z)
+// of corner k.  -pi/2 where the triangle does not reach it.
+double seg_max_elev(const double P[3][3], double R2) {
+    double pts[3][2];
+    int np = 0;
+    for (int k = 0; k < 3 && np < 3; k++) {
+        const double* A = P[k];
+        const double* B = P[(k + 1) % 3];
+        if (A[1] == 0.0) {
+            pts[np][0] = A[0];
+            pts[np][1] = A[2];
+            np++;
+        }
+        if ((A[1] < 0.0 && B[1] > 0.0) || (A[1] > 0.0 && B[1] < 0.0)) {
+            double s = A[1] / (A[1] - B[1]);
+            pts[np][0] = A[0] + s * (B[0] - A[0]);
+            pts[np][1] = A[2] + s * (B[2] - A[2]);
+            np++;
+        }
+    }
+    const double none = -kPi / 2;
+    if (np < 2) return none;
+    double a0 = pts[0][0], z0 = pts[0][1], a1 = pts[1][0], z1 = pts[1][1];
+    // keep u >= 0
+    if (a0 < 0 && a1 < 0) return none;
+    if (a0 < 0 || a1 < 0) {
+        double s = a0 / (a0 - a1);
+        double zc = z0 + s * (z1 - z0);
+        if (a0 < 0) {
+            a0 = 0.0;
+            z0 = zc;
+        } else {
+            a1 = 0.0;
+            z1 = zc;
+        }
+    }
+    // clip to the search radius: |(u, z)| <= tfar
+    auto inside = [&](double u, double z) { return u * u + z * z <= R2; };
+    bool i0 = inside(a0, z0), i1 = inside(a1, z1);
+    if (!i0 && !i1) {
+        // the segment may still cross the disc; test closest point
+        double du = a1 - a0, dz = z1 - z0;
+        double L2 = du * du + dz * dz;
+        if (L2 == 0) return none;
+        double s = -(a0 * du + z0 * dz) / L2;
+        if (s <= 0 || s >= 1) return none;
+        double cu = a0 + s * du, cz = z0 + s * dz;
+        if (!inside(cu, cz)) return none;
+        // both ends outside: clip both
+        double disc = std::sqrt(std::max(0.0, R2 - (cu * cu + cz * cz)) / L2);
+        double s0 = s - disc, s1 = s + disc;
+        double na0 = a0 + s0 * du, nz0 = z0 + s0 * dz;
+        double na1 = a0 + s1 * du, nz1 = z0 + s1 * dz;
+        a0 = na0;
+        z0 = nz0;
+        a1 = na1;
+        z1 = nz1;
+    } else if (!i0 || !i1) {
+        if (!i0) {
+            std::swap(a0, a1);
+            std::swap(z0, z1);
+        }
+        double du = a1 - a0, dz = z1 - z0;
+        double A2 = du * du + dz * dz;
+        double Bq = 2 * (a0 * du + z0 * dz);
+        double Cq = a0 * a0 + z0 * z0 - R2;
+        double s = (-Bq + std::sqrt(std::max(0.0, Bq * Bq - 4 * A2 * Cq))) / (2 * A2);
+        a1 = a0 + s * du;
+        z1 = z0 + s * dz;
+    }
+    double e0 = std::atan2(z0, a0), e1 = std::atan2(z1, a1);
+    return std::max(e0, e1);
+}
+
+}  // namespace
+
 int orc_true_horizon(const double* clon, const double* clat, const double* hsurf,
                      const double* vlon, const double* vlat, const int* cov, int ncell, int nvert,
                      int nhori, int grid_type, double dist_search, double ray_org_elev, int rf,
-                     const int* cells, int ncells_req, double* out, int nthreads) {
+                     const int* cells, int ncells_req, double* out, int nthreads, int brute) {
     Setup S;
     setup_domain(S, clon, clat, hsurf, vlon, vlat, cov, ncell, nvert, nhori, grid_type,
                  dist_search, ray_org_elev, rf, 2, false);
     if (nthreads <= 0) nthreads = (int)std::max(1u, std::thread::hardware_concurrency());
     std::atomic<int> next(0);
     const double R2 = (double)S.tfar * (double)S.tfar;
+    const int ntri = S.scene.ntri;
+    const int nmv = (int)(S.scene.vx.size() / 3);
+    // triangle bounding boxes (FP64 of the FP32 corners)
+    std::vector<double> box((size_t)ntri * 6);
+    for (int t = 0; t < ntri; t++) {
+        double* bx = &box[(size_t)t * 6];
+        for (int a = 0; a < 3; a++) {
+            bx[a] = 1e300;
+            bx[3 + a] = -1e300;
+        }
+        for (int k = 0; k < 3; k++) {
+            const float* v = &S.scene.vx[3 * S.scene.tri[3 * t + k]];
+            for (int a = 0; a < 3; a++) {
+                bx[a] = std::min(bx[a], (double)v[a]);
+                bx[3 + a] = std::max(bx[3 + a], (double)v[a]);
+            }
+        }
+    }
     auto worker = [&]() {
+        std::vector<double> ang;            // horizontal angle of each vertex from the origin
+        std::vector<unsigned char> axis;    // vertex within 1 m of the origin's vertical axis
+        std::vector<std::vector<int>> bin(S.L);
         while (true) {
             int q = next.fetch_add(1);
             if (q >= ncells_req) break;
@@ -985,6 +1096,71 @@ int orc_true_horizon(const double* clon, const double* clat, const double* hsurf
             double shift = 0.0;
             if (S.rf != 1)
                 shift = -(deg2rad(360.0) / (2.0 * S.nhori)) + (deg2rad(360.0) / (2.0 * (double)S.L));
+            const double dphi = 2.0 * kPi / S.L;
+            auto tri_elev = [&](int t, const V3& hdir, const V3& side) {
+                double P[3][3];  // (u, w, z) per corner
+                for (int k = 0; k < 3; k++) {
+                    const float* v = &S.scene.vx[3 * S.scene.tri[3 * t + k]];
+                    V3 r = {v[0] - o[0], v[1] - o[1], v[2] - o[2]};
+                    P[k][0] = dot3(r, hdir);
+                    P[k][1] = dot3(r, side);
+                    P[k][2] = dot3(r, f.n);
+                }
+                return seg_max_elev(P, R2);
+            };
+            if (!brute) {
+                ang.resize(nmv);
+                axis.resize(nmv);
+                for (int v = 0; v < nmv; v++) {
+                    const float* p = &S.scene.vx[3 * v];
+                    V3 r = {p[0] - o[0], p[1] - o[1], p[2] - o[2]};
+                    const double U = dot3(r, f.north), W = dot3(r, east);
+                    ang[v] = std::atan2(W, U);
+                    axis[v] = (U * U + W * W) < 1.0;  // within 1 m: every azimuth
+                }
+                for (auto& b : bin) b.clear();
+                const double eps = 1e-9;
+                for (int t = 0; t < ntri; t++) {
+                    const double* bx = &box[(size_t)t * 6];
+                    double d2 = 0.0;
+                    for (int a = 0; a < 3; a++) {
+                        const double d = std::max(std::max(bx[a] - o[a], o[a] - bx[3 + a]), 0.0);
+                        d2 += d * d;
+                    }
+                    if (d2 > R2) continue;
+                    const int* tv = &S.scene.tri[3 * t];
+                    bool all = axis[tv[0]] || axis[tv[1]] || axis[tv[2]];
+                    double lo = 0.0, span = 2.0 * kPi;
+                    if (!all) {
+                        // the arc the three corner angles span: the complement of the
+                        // largest gap between them; a gap of at most pi means the
+                        // projection may hold the origin, which every azimuth meets
+                        double a3[3] = {ang[tv[0]], ang[tv[1]], ang[tv[2]]};
+                        std::sort(a3, a3 + 3);
+                        const double g[3] = {a3[1] - a3[0], a3[2] - a3[1], a3[0] + 2.0 * kPi - a3[2]};
+                        int m = 0;
+                        for (int k = 1; k < 3; k++)
+                            if (g[k] > g[m]) m = k;
+                        if (g[m] <= kPi + 1e-6) {
+                            all = true;
+                        } else {
+                            lo = a3[(m + 1) % 3];
+                            span = 2.0 * kPi - g[m];
+                        }
+                    }
+                    if (all) {
+                        for (int j = 0; j < S.L; j++) bin[j].push_back(t);
+                        continue;
+                    }
+                    const long k0 = (long)std::ceil((lo - eps - shift) / dphi);
+                    const long k1 = (long)std::floor((lo + span + eps - shift) / dphi);
+                    if (k1 - k0 + 1 >= S.L) {
+                        for (int j = 0; j < S.L; j++) bin[j].push_back(t);
+                        continue;
+                    }
+                    for (long k = k0; k <= k1; k++) bin[(int)(((k % S.L) + S.L) % S.L)].push_back(t);
+                }
+            }
             for (int j = 0; j < S.L; j++) {
                 double phi = shift + j * (2.0 * kPi / S.L);
                 V3 hdir = {std::cos(phi) * f.north.x + std::sin(phi) * east.x,
@@ -992,84 +1168,10 @@ int orc_true_horizon(const double* clon, const double* clat, const double* hsurf
                            std::cos(phi) * f.north.z + std::sin(phi) * east.z};
                 V3 side = cross3(f.n, hdir);  // normal of the vertical plane
                 double best = -kPi / 2;
-                for (int t = 0; t < S.scene.ntri; t++) {
-                    double P[3][3];  // (u, w, z) per corner
-                    for (int k = 0; k < 3; k++) {
-                        const float* v = &S.scene.vx[3 * S.scene.tri[3 * t + k]];
-                        V3 r = {v[0] - o[0], v[1] - o[1], v[2] - o[2]};
-                        P[k][0] = dot3(r, hdir);
-                        P[k][1] = dot3(r, side);
-                        P[k][2] = dot3(r, f.n);
-                    }
-                    // segment where the triangle meets w = 0
-                    double pts[3][2];
-                    int np = 0;
-                    for (int k = 0; k < 3 && np < 3; k++) {
-                        const double* A = P[k];
-                        const double* B = P[(k + 1) % 3];
-                        if (A[1] == 0.0) {
-                            pts[np][0] = A[0];
-                            pts[np][1] = A[2];
-                            np++;
-                        }
-                        if ((A[1] < 0.0 && B[1] > 0.0) || (A[1] > 0.0 && B[1] < 0.0)) {
-                            double s = A[1] / (A[1] - B[1]);
-                            pts[np][0] = A[0] + s * (B[0] - A[0]);
-                            pts[np][1] = A[2] + s * (B[2] - A[2]);
-                            np++;
-                        }
-                    }
-                    if (np < 2) continue;
-                    double a0 = pts[0][0], z0 = pts[0][1], a1 = pts[1][0], z1 = pts[1][1];
-                    // keep u >= 0
-                    if (a0 < 0 && a1 < 0) continue;
-                    if (a0 < 0 || a1 < 0) {
-                        double s = a0 / (a0 - a1);
-                        double zc = z0 + s * (z1 - z0);
-                        if (a0 < 0) {
-                            a0 = 0.0;
-                            z0 = zc;
-                        } else {
-                            a1 = 0.0;
-                            z1 = zc;
-                        }
-                    }
-                    // clip to the search radius: |(u, z)| <= tfar
-                    auto inside = [&](double u, double z) { return u * u + z * z <= R2; };
-                    bool i0 = inside(a0, z0), i1 = inside(a1, z1);
-                    if (!i0 && !i1) {
-                        // the segment may still cross the disc; test closest point
-                        double du = a1 - a0, dz = z1 - z0;
-                        double L2 = du * du + dz * dz;
-                        if (L2 == 0) continue;
-                        double s = -(a0 * du + z0 * dz) / L2;
-                        if (s <= 0 || s >= 1) continue;
-                        double cu = a0 + s * du, cz = z0 + s * dz;
-                        if (!inside(cu, cz)) continue;
-                        // both ends outside: clip both
-                        double disc = std::sqrt(std::max(0.0, R2 - (cu * cu + cz * cz)) / L2);
-                        double s0 = s - disc, s1 = s + disc;
-                        double na0 = a0 + s0 * du, nz0 = z0 + s0 * dz;
-                        double na1 = a0 + s1 * du, nz1 = z0 + s1 * dz;
-                        a0 = na0;
-                        z0 = nz0;
-                        a1 = na1;
-                        z1 = nz1;
-                    } else if (!i0 || !i1) {
-                        if (!i0) {
-                            std::swap(a0, a1);
-                            std::swap(z0, z1);
-                        }
-                        double du = a1 - a0, dz = z1 - z0;
-                        double A2 = du * du + dz * dz;
-                        double Bq = 2 * (a0 * du + z0 * dz);
-                        double Cq = a0 * a0 + z0 * z0 - R2;
-                        double s = (-Bq + std::sqrt(std::max(0.0, Bq * Bq - 4 * A2 * Cq))) / (2 * A2);
-                        a1 = a0 + s * du;
-                        z1 = z0 + s * dz;
-                    }
-                    double e0 = std::atan2(z0, a0), e1 = std::atan2(z1, a1);
-                    best = std::max(best, std::max(e0, e1));
+                if (brute) {
+                    for (int t = 0; t < ntri; t++) best = std::max(best, tri_elev(t, hdir, side));
+                } else {
+                    for (int t : bin[j]) best = std::max(best, tri_elev(t, hdir, side));
                 }
                 out[(size_t)q * S.L + j] = best;
             }

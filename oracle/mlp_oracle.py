@@ -79,3 +79,26 @@ def gradients(p, X, T, bf=False):
     g["w1"] = dZ1.T @ f["X"]
     g["b1"] = dZ1.sum(0)
     return g, float(np.mean((f["Y"] - T) ** 2))
+
+
+# Agreement bars of a device backward pass with gradients(..., bf=True):
+# per tensor, the error norm relative to the reference norm and the largest
+# element error relative to the largest reference element.
+GRAD_REL_TOL = 2e-3
+GRAD_MAX_TOL = 1e-2
+
+
+def gradient_errors(g, gr):
+    """{name: (|g - gr| / |gr|, max|g - gr| / max|gr|)} per parameter tensor."""
+    out = {}
+    for name, _ in SHAPES:
+        a = np.asarray(g[name], np.float64)
+        b = np.asarray(gr[name], np.float64)
+        nb = max(float(np.linalg.norm(b)), 1e-30)
+        mb = max(float(np.abs(b).max()), 1e-30)
+        out[name] = (float(np.linalg.norm(a - b)) / nb, float(np.abs(a - b).max()) / mb)
+    return out
+
+
+def gradients_agree(g, gr):
+    return all(r <= GRAD_REL_TOL and m <= GRAD_MAX_TOL for r, m in gradient_errors(g, gr).values())

@@ -155,8 +155,11 @@ def occluded64(vert, tri, org, dirs, tfar):
 
 
 def true_horizon(dom, cells, num_hori=24, grid_type=1, dist_search=40000.0,
-                 ray_org_elev=0.2, refine_factor=1, nthreads=0):
-    """FP64 half-plane maximum elevation angle [rad], shape (len(cells), L)."""
+                 ray_org_elev=0.2, refine_factor=1, nthreads=0, brute=False):
+    """FP64 half-plane maximum elevation angle [rad], shape (len(cells), L).
+    brute=True evaluates every triangle at every azimuth; the default visits
+    only the triangles that can meet each azimuth's half-plane within
+    dist_search (same bits)."""
     clon, clat, hsurf, vlon, vlat, cov = _inputs(dom)
     cells = np.ascontiguousarray(cells, np.int32)
     L = num_hori * refine_factor
@@ -164,5 +167,5 @@ def true_horizon(dom, cells, num_hori=24, grid_type=1, dist_search=40000.0,
     lib().orc_true_horizon(_p(clon, _D), _p(clat, _D), _p(hsurf, _D), _p(vlon, _D),
                            _p(vlat, _D), _p(cov, _I), clon.size, vlon.size, num_hori,
                            grid_type, _D(dist_search), _D(ray_org_elev), refine_factor,
-                           _p(cells, _I), cells.size, _p(out, _D), nthreads)
+                           _p(cells, _I), cells.size, _p(out, _D), nthreads, int(brute))
     return out

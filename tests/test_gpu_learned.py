@@ -47,7 +47,8 @@ def test_forward_matches_oracle():
     ref = mo.forward(mo.unflatten(w), learned.bf16_to_f32(Xb), bf=True)["Y"]
     err = np.abs(Y - ref).max()
     print(f"forward max|dy|={err:.3g} (max|y|={np.abs(ref).max():.3g})", flush=True)
-    assert err <= 2e-2 * np.abs(ref).max() + 1e-3
+    # BF16 products with FP32 accumulation against the same products in FP64
+    assert err <= 2e-3 * np.abs(ref).max()
 
 
 def test_gradients_match_oracle():
@@ -59,10 +60,12 @@ def test_gradients_match_oracle():
     gr, lref = mo.gradients(mo.unflatten(w), learned.bf16_to_f32(Xb), T, bf=True)
     gg = learned.unflatten(g)
     assert abs(loss - lref) <= 1e-3 * lref
-    for name, _ in mo.SHAPES:
-        r = _rel(gg[name], gr[name])
-        print(f"grad {name}: rel err {r:.2e}", flush=True)
-        assert r < 3e-2, name
+    for name, (r, m) in mo.gradient_errors(gg, gr).items():
+        print(f"grad {name}: rel err {r:.2e}, max err {m:.2e}", flush=True)
+    # per tensor: error norm <= 2e-3 of the reference norm, largest element
+    # error <= 1e-2 of the largest reference element (the bar a dropped GELU
+    # derivative term on one row in eight fails, tests/test_mlp_oracle.py)
+    assert mo.gradients_agree(gg, gr), mo.gradient_errors(gg, gr)
 
 
 def test_inference_rows_independent_of_batch():

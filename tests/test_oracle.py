@@ -323,6 +323,18 @@ def test_true_horizon_quantisation(c1):
     assert agree[clear].all()
 
 
+@pytest.mark.parametrize("dist,num_hori,rf", [(40000.0, 24, 1), (3000.0, 12, 3)])
+def test_true_horizon_cull_equals_brute_force(c1, dist, num_hori, rf):
+    # the azimuth-arc and search-ball cull visits a subset of the triangles
+    # with the same arithmetic: the same bits as every triangle at every azimuth
+    cells = np.arange(0, c1.ncell, 47)
+    a = oracle.true_horizon(c1, cells, num_hori=num_hori, refine_factor=rf,
+                            dist_search=dist, brute=True)
+    b = oracle.true_horizon(c1, cells, num_hori=num_hori, refine_factor=rf, dist_search=dist)
+    assert np.array_equal(a, b)
+    assert (a <= -np.pi / 2 + 1e-12).any() and (a > 0).any()
+
+
 def test_walk_equals_climb_off_border(c1):
     h0, _, s0 = oracle.horizon_svf(c1, grid_type=1, mode=0)
     h1, _, s1 = oracle.horizon_svf(c1, grid_type=1, mode=1)
