@@ -849,10 +849,14 @@ __global__ __launch_bounds__(BLOCK, 4) void k_trace_pt(const BvhNode* __restrict
                 D3 ax = {0.0, 0.0, 0.0};
                 double e = 0.0;
                 if (MODE == 0) {
-                    ax = {SV(S_AX), SV(S_AX + 1), SV(S_AX + 2)};
+                    if (phase != P_AZSTART) ax = {SV(S_AX), SV(S_AX + 1), SV(S_AX + 2)};
                     e = SV(S_E);
                 }
-                double sector = SV(S_SECTOR), svf = SV(S_SVF);
+                // the per-azimuth fields (sector sums, azimuth counters, axis) are
+                // read when an azimuth ends and written back only when one ended
+                // or started here; a step of a run rewrites dir, e and the phase
+                double sector = 0.0, svf = 0.0;
+                bool az_read = false, az_dirty = false;
                 D3 q = dir;  // direction of the query to issue
                 bool issue = false;
                 // the last occluder, fetched now so that the load overlaps the search
@@ -873,6 +877,7 @@ __global__ __launch_bounds__(BLOCK, 4) void k_trace_pt(const BvhNode* __restrict
                     double h = 0.0;
                     if (phase == P_AZSTART) {
                         {
+                            az_dirty = true;
                             phase = P_FIRST;
                             if (MODE == 0) {
                                 ax = unit_cross(dir, n);
@@ -965,6 +970,12 @@ __global__ __launch_bounds__(BLOCK, 4) void k_trace_pt(const BvhNode* __restrict
                         has_cell = false;
                         break;
                     }
+                    if (!az_read) {
+                        sector = SV(S_SECTOR);
+                        svf = SV(S_SVF);
+                        az_read = true;
+                    }
+                    az_dirty = true;
                     sector += h;
                     if (++ksec == P.refine_factor) {
                         lhorizon[(size_t)jsec * lncell + cell] =
@@ -1004,22 +1015,34 @@ __global__ __launch_bounds__(BLOCK, 4) void k_trace_pt(const BvhNode* __restrict
                 }
                 if (issue) {
                     SI(S_I0) = make_int2(cell, phase);
-                    SI(S_I1) = make_int2(j, kk);
-                    SI(S_I2) = make_int2(ksec, jsec);
                     node = kWideBit;  // the root is read in full
                     sp = 0;
                     steps = 0;
                     live = true;
-                    SV(S_DIR) = dir.x;
-                    SV(S_DIR + 1) = dir.y;
-                    SV(S_DIR + 2) = dir.z;
-                    SV(S_SECTOR) = sector;
-                    SV(S_SVF) = svf;
                     if (MODE == 0) {
-                        SV(S_AX) = ax.x;
-                        SV(S_AX + 1) = ax.y;
-                        SV(S_AX + 2) = ax.z;
+                        SV(S_DIR) = dir.x;
+                        SV(S_DIR + 1) = dir.y;
+                        SV(S_DIR + 2) = dir.z;
                         SV(S_E) = e;
+                    } else {
+                        SI(S_I1) = make_int2(j, kk);  // the lattice index moves every query
+                    }
+                    if (az_dirty) {
+                        if (MODE == 0) {
+                            SI(S_I1) = make_int2(j, kk);
+                            SV(S_AX) = ax.x;
+                            SV(S_AX + 1) = ax.y;
+                            SV(S_AX + 2) = ax.z;
+                        } else {
+                            SV(S_DIR) = dir.x;
+                            SV(S_DIR + 1) = dir.y;
+                            SV(S_DIR + 2) = dir.z;
+                        }
+                        SI(S_I2) = make_int2(ksec, jsec);
+                        if (az_read) {
+                            SV(S_SECTOR) = sector;
+                            SV(S_SVF) = svf;
+                        }
                     }
                 }
             }
