@@ -35,7 +35,7 @@ class HzOptions(ctypes.Structure):
                 ("bvh_morton", ctypes.c_int), ("queue_chunk", ctypes.c_int),
                 ("bvh_width", ctypes.c_int), ("prologue", ctypes.c_int),
                 ("whole_cells", ctypes.c_int), ("certify", ctypes.c_int),
-                ("reserved", ctypes.c_int * 2)]
+                ("cache", ctypes.c_int), ("reserved", ctypes.c_int * 1)]
 
 
 class HzStats(ctypes.Structure):
@@ -72,6 +72,7 @@ SIGNATURES = [
               ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_char_p,
               c_int_p]),
     ("hz_last_error", ctypes.c_char_p, []),
+    ("hz_cache_release", None, []),
     ("hz_default_options", None, [ctypes.POINTER(HzOptions)]),
     ("hz_domain_create", ctypes.c_int,
      _ARR6 + [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,

@@ -5,9 +5,10 @@
 // reference has no C-side validation and no status; this layer validates the
 // same ranges the reference's Python binding checks (horizon_svf.pyx:79-103),
 // returns a status from the _ex / hz_domain_* entry points and fills the
-// outputs with NaN on failure.  There is no global mutable state besides the
-// per-thread error message, so calls are reentrant (each call owns its
-// stream and buffers).
+// outputs with NaN on failure.  Global mutable state: the per-thread error
+// message, and the one-shot calls' buffer cache, whose mutex hands a kept
+// domain to one call at a time; calls are reentrant (each call owns its
+// stream and buffers while it runs).
 
 #include <hip/hip_runtime.h>
 
@@ -18,6 +19,7 @@
 #include <cstddef>
 #include <cstring>
 #include <limits>
+#include <mutex>
 #include <vector>
 
 #include "../../include/horayzon.h"
@@ -151,6 +153,7 @@ struct hz_domain {
     hipStream_t pstream = nullptr;
     hipEvent_t pev[3] = {nullptr, nullptr, nullptr};  // fork, predictor start, join
     bool ran = false;
+    bool host_scene = false;  // built from a host prologue (the one-shot calls' cache holds these)
 };
 
 // The progress report the reference prints to stdout (mo_lradtopo_horayzon.cpp
@@ -221,6 +224,7 @@ void hz_default_options(hz_options* o) {
     o->prologue = HZ_PROLOGUE_HOST;
     o->whole_cells = 0;
     o->certify = 1;
+    o->cache = 1;
 }
 
 void hz_domain_destroy(hz_domain* d) {
@@ -304,6 +308,11 @@ long long hz_domain_num_triangles(const hz_domain* d) { return d ? d->ntri : 0; 
         }                                                                              \
     } while (0)
 
+static int make_domain(const double* clon, const double* clat, const double* hsurf,
+                       const double* vlon, const double* vlat, const int* cov, int num_cell,
+                       int num_vertex, int grid_type, double ray_org_elev, int where,
+                       const HostScene& hs, double t0, hz_domain** out, hz_stats* stats);
+
 int hz_domain_create(const double* clon, const double* clat, const double* hsurf,
                      const double* vlon, const double* vlat, const int* cov, int num_cell,
                      int num_vertex, int grid_type, double ray_org_elev, const hz_options* opt,
@@ -326,6 +335,18 @@ int hz_domain_create(const double* clon, const double* clat, const double* hsurf
                       ray_org_elev, hs, err);
         if (rc) return fail(HZ_EINVAL, err);
     }
+    return make_domain(clon, clat, hsurf, vlon, vlat, cov, num_cell, num_vertex, grid_type,
+                       ray_org_elev, where, hs, t0, out, stats);
+}
+
+// The device side of hz_domain_create: buffers, streams and events, then the
+// host scene's upload (or the device prologue).
+static int make_domain(const double* clon, const double* clat, const double* hsurf,
+                       const double* vlon, const double* vlat, const int* cov, int num_cell,
+                       int num_vertex, int grid_type, double ray_org_elev, int where,
+                       const HostScene& hs, double t0, hz_domain** out, hz_stats* stats) {
+    int rc = 0;
+    std::string err;
     double t1 = now_ms();
     hz_domain* d = new hz_domain();
     {
@@ -364,6 +385,7 @@ int hz_domain_create(const double* clon, const double* clat, const double* hsurf
         std::memcpy(d->bounds_xy, ds.bounds_xy, sizeof(d->bounds_xy));
         t1 = now_ms();
     } else {
+        d->host_scene = true;
         d->ntri = (long long)hs.tri.size() / 3;
         d->nmeshvert = hs.num_mesh_vertices;
         d->lon0 = hs.lon0;
@@ -938,6 +960,148 @@ int hz_domain_fetch(hz_domain* d, double* horizon, double* skyview, hz_stats* st
     return HZ_OK;
 }
 
+// Device-buffer cache of the one-shot calls (horizon_svf_comp and its
+// variants, option cache = 1): per device, the domain of the last call is
+// kept, buffers, streams, events and pinned staging included, and the next
+// call whose host prologue gives the same sizes (cells, triangles, mesh
+// vertices) loads its scene into them instead of allocating.  The kept
+// domain is taken out of the cache while a call uses it, so concurrent calls
+// never share one.  hz_cache_release frees it.
+namespace {
+constexpr int kCacheDevices = 64;
+std::mutex g_cache_mu;
+hz_domain* g_cache[kCacheDevices] = {};
+
+// The kept domain of device dev if its sizes match (removed from the
+// cache); a kept domain of other sizes is freed first, so that the call's
+// own allocation does not have to fit beside it.
+hz_domain* cache_take(int dev, int ncell, long long ntri, long long nmv) {
+    if (dev < 0 || dev >= kCacheDevices) return nullptr;
+    hz_domain* d = nullptr;
+    {
+        std::lock_guard<std::mutex> g(g_cache_mu);
+        d = g_cache[dev];
+        g_cache[dev] = nullptr;
+    }
+    if (d && (d->ncell != ncell || d->ntri != ntri || d->nmeshvert != nmv || !d->host_scene)) {
+        hz_domain_destroy(d);
+        d = nullptr;
+    }
+    return d;
+}
+
+// The host scene arrays of the last one-shot call (about 150 bytes per
+// cell), kept so that the next prologue writes into memory that is already
+// mapped instead of faulting in fresh pages.
+std::mutex g_host_mu;
+HostScene* g_host_kept = nullptr;
+
+HostScene* host_take() {
+    HostScene* h = nullptr;
+    {
+        std::lock_guard<std::mutex> g(g_host_mu);
+        h = g_host_kept;
+        g_host_kept = nullptr;
+    }
+    return h ? h : new HostScene();
+}
+
+void host_put(HostScene* h) {
+    {
+        std::lock_guard<std::mutex> g(g_host_mu);
+        if (!g_host_kept) {
+            g_host_kept = h;
+            h = nullptr;
+        }
+    }
+    delete h;
+}
+
+// Keep d for the next call on its device (a domain kept meanwhile by a
+// concurrent call is freed).
+void cache_put(hz_domain* d) {
+    if (!d || !d->host_scene || d->device < 0 || d->device >= kCacheDevices) {
+        hz_domain_destroy(d);
+        return;
+    }
+    hz_domain* old = nullptr;
+    {
+        std::lock_guard<std::mutex> g(g_cache_mu);
+        old = g_cache[d->device];
+        g_cache[d->device] = d;
+    }
+    hz_domain_destroy(old);
+}
+
+// Load a host scene of the same sizes into a kept domain: the arrays are
+// uploaded, and what was derived from the previous scene (certificate index,
+// slope adjacency, predictor, cell flags, run state) is dropped.
+int domain_reload(hz_domain* d, const HostScene& hs, const double* hsurf, int num_cell,
+                  int grid_type, hz_stats* stats, double t0, double t1) {
+    hipError_t e = hipSetDevice(d->device);
+    d->ran = false;
+    d->last_nhori = 0;
+    d->cert.dist_key = -1.0f;
+    d->near_flags = false;
+    d->predictor = nullptr;
+    d->have_pred = false;
+    if (e == hipSuccess && d->d_adj_off) e = hipFree(d->d_adj_off);
+    if (e == hipSuccess && d->d_adj) e = hipFree(d->d_adj);
+    d->d_adj_off = nullptr;
+    d->d_adj = nullptr;
+    d->grid_type = grid_type;
+    d->hmin = d->hmax = hsurf[0];
+    for (int i = 1; i < num_cell; i++) {
+        d->hmin = std::min(d->hmin, hsurf[i]);
+        d->hmax = std::max(d->hmax, hsurf[i]);
+    }
+    d->lon0 = hs.lon0;
+    d->lat0 = hs.lat0;
+    std::memcpy(d->bounds_xy, hs.bounds_xy, sizeof(d->bounds_xy));
+    const size_t nv = hs.vert32.size(), nt = hs.tri.size();
+    if (e == hipSuccess && nv)
+        e = d->stager.copy(d->d_vert, hs.vert32.data(), nv * sizeof(float), true, d->stream);
+    if (e == hipSuccess && nt)
+        e = d->stager.copy(d->d_tri, hs.tri.data(), nt * sizeof(int), true, d->stream);
+    if (e == hipSuccess)
+        e = d->stager.copy(d->d_cells, hs.frame.data(), (size_t)num_cell * sizeof(CellFrame), true,
+                           d->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(d->stream);
+    if (e != hipSuccess)
+        return fail(e == hipErrorOutOfMemory ? HZ_ENOMEM : HZ_EDEVICE,
+                    std::string("reloading a kept domain: ") + hipGetErrorString(e));
+    if (stats) {
+        std::memset(stats, 0, sizeof(*stats));
+        stats->num_triangles = d->ntri;
+        stats->num_mesh_vertices = d->nmeshvert;
+        stats->t_prologue_ms = t1 - t0;
+        stats->t_upload_ms = now_ms() - t1;
+        stats->lon0 = d->lon0;
+        stats->lat0 = d->lat0;
+    }
+    return HZ_OK;
+}
+}  // namespace
+
+void hz_cache_release(void) {
+    hz_domain* kept[kCacheDevices];
+    {
+        std::lock_guard<std::mutex> g(g_cache_mu);
+        for (int i = 0; i < kCacheDevices; i++) {
+            kept[i] = g_cache[i];
+            g_cache[i] = nullptr;
+        }
+    }
+    for (hz_domain* d : kept) hz_domain_destroy(d);
+    HostScene* h = nullptr;
+    {
+        std::lock_guard<std::mutex> g(g_host_mu);
+        h = g_host_kept;
+        g_host_kept = nullptr;
+    }
+    delete h;
+}
+
 int horizon_svf_comp_ex(const double* clon, const double* clat, const double* hsurf,
                         const double* vlon, const double* vlat, const int* cov, double* horizon,
                         double* skyview, int num_cell, int num_vertex, int num_hori,
@@ -967,8 +1131,42 @@ int horizon_svf_comp_ex(const double* clon, const double* clat, const double* hs
     hz_stats* s = stats ? stats : &local;
     std::memset(s, 0, sizeof(*s));
     hz_domain* d = nullptr;
-    rc = hz_domain_create(clon, clat, hsurf, vlon, vlat, cov, num_cell, num_vertex, grid_type,
-                          ray_org_elev, &o, &d, s);
+    if (o.cache != 0 && o.cache != 1) rc = fail(HZ_EINVAL, "cache must be 0 or 1");
+    if (!rc && o.cache == 1 && o.prologue == HZ_PROLOGUE_HOST) {
+        // host prologue, then the kept domain of these sizes if there is one
+        rc = clon && clat && hsurf && (num_vertex <= 0 || (vlon && vlat && cov))
+                 ? check_grid_args(cov, num_cell, num_vertex, grid_type, ray_org_elev)
+                 : fail(HZ_EINVAL, "NULL input array");
+        HostScene* hsp = host_take();
+        HostScene& hs = *hsp;
+        std::string err;
+        const double tp0 = now_ms();
+        if (!rc) {
+            rc = prologue(clon, clat, hsurf, vlon, vlat, cov, num_cell, num_vertex, grid_type,
+                          ray_org_elev, hs, err);
+            if (rc) rc = fail(HZ_EINVAL, err);
+        }
+        if (!rc) {
+            const double tp1 = now_ms();
+            int dev = 0;
+            if (hipGetDevice(&dev) != hipSuccess) dev = -1;
+            d = cache_take(dev, num_cell, (long long)hs.tri.size() / 3, hs.num_mesh_vertices);
+            if (d) {
+                rc = domain_reload(d, hs, hsurf, num_cell, grid_type, s, tp0, tp1);
+                if (rc) {
+                    hz_domain_destroy(d);
+                    d = nullptr;
+                }
+            } else {
+                rc = make_domain(clon, clat, hsurf, vlon, vlat, cov, num_cell, num_vertex,
+                                 grid_type, ray_org_elev, HZ_PROLOGUE_HOST, hs, tp0, &d, s);
+            }
+        }
+        host_put(hsp);
+    } else if (!rc) {
+        rc = hz_domain_create(clon, clat, hsurf, vlon, vlat, cov, num_cell, num_vertex,
+                              grid_type, ray_org_elev, &o, &d, s);
+    }
     if (rc) {
         fill_nan(horizon, skyview, num_hori, num_cell);
         return rc;
@@ -982,7 +1180,10 @@ int horizon_svf_comp_ex(const double* clon, const double* clat, const double* hs
     rc = hz_domain_run(d, num_hori, dist_search, refine_factor, svf_type, &o, s);
     if (rc == 0) rc = hz_domain_fetch(d, horizon, skyview, s);
     std::string keep = g_err;
-    hz_domain_destroy(d);
+    if (rc == 0 && o.cache == 1)
+        cache_put(d);
+    else
+        hz_domain_destroy(d);
     g_err = keep;
     s->t_prologue_ms = tp;
     s->t_upload_ms = tu;

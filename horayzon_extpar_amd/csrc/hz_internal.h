@@ -202,6 +202,9 @@ struct HostScene {
     long long num_mesh_vertices = 0;
     double lon0 = 0.0, lat0 = 0.0;
     float bounds_xy[4] = {0.f, 0.f, 0.f, 0.f};  // x / y extent of vert32 (min x, min y, max x, max y)
+    HostVec<double> work;   // FP64 mesh vertices (lon/lat/h, then ECEF, then ENU)
+    HostVec<int> vcount;    // per ICON vertex: triangles, added mesh vertices
+    HostVec<long long> voff;  // their prefix sums
 };
 
 int prologue(const double* clon, const double* clat, const double* hsurf, const double* vlon,

@@ -56,29 +56,61 @@ void parallel_for(long long n, F&& f) {
 // Valid adjacent cells of ICON vertex v, ordered clockwise from the
 // latitude axis (reference :243-261; for <= 6 keys std::sort keeps equal
 // angles in input order, which this insertion sort reproduces).
+//
+// The reference sorts by atan2(dlon, dlat) mapped to [0, 2 pi).  Only the
+// order matters, so the directions are compared exactly instead: by the
+// half-turn they lie in ([0, pi): dlon > 0, or dlon == 0 and dlat >= 0),
+// then by the sign of their cross product.  Where directions lie closer than
+// 1e-9 rad to each other or to the half-turn boundary, the rounding of
+// atan2 could order them otherwise (or tie them), and the atan2 keys decide.
 inline int sorted_cells(const double* clon, const double* clat, const double* vlon,
                         const double* vlat, const int* cov, int nvert, int v, int* cell,
                         double* hsum, const double* hsurf) {
-    double ang[6];
+    double dy[6], dx[6], r2[6];
+    int half[6];
     int c6[6];
     int n = 0;
     double hs = 0.0;
+    bool exact = true;
+    constexpr double kSep = 1e-9;
     for (int j = 0; j < 6; j++) {
         int c = cov[(size_t)nvert * j + v];
         if (c == -2) continue;
-        double a = std::atan2(clon[c] - vlon[v], clat[c] - vlat[v]);
-        if (a < 0.0) a += 2.0 * kPi;
-        ang[n] = a;
+        const double y = clon[c] - vlon[v], x = clat[c] - vlat[v];
+        dy[n] = y;
+        dx[n] = x;
+        r2[n] = x * x + y * y;
+        half[n] = (y > 0.0 || (y == 0.0 && x >= 0.0)) ? 0 : 1;
+        if (!(std::fabs(y) > kSep * std::sqrt(r2[n]))) exact = false;
         c6[n] = c;
         hs += hsurf[c];
         n++;
     }
+    for (int i = 0; exact && i < n; i++)
+        for (int k = i + 1; k < n; k++)
+            if (half[i] == half[k] &&
+                !(std::fabs(dx[i] * dy[k] - dy[i] * dx[k]) > kSep * std::sqrt(r2[i] * r2[k]))) {
+                exact = false;
+                break;
+            }
+    double ang[6];
+    if (!exact)
+        for (int i = 0; i < n; i++) {
+            double a = std::atan2(dy[i], dx[i]);
+            if (a < 0.0) a += 2.0 * kPi;
+            ang[i] = a;
+        }
+    auto before = [&](int a, int b) {
+        if (!exact) return ang[a] < ang[b];
+        if (half[a] != half[b]) return half[a] < half[b];
+        return dx[a] * dy[b] - dy[a] * dx[b] > 0.0;
+    };
     int idx[6];
     for (int i = 0; i < n; i++) idx[i] = i;
     for (int i = 1; i < n; i++) {
         int t = idx[i];
         int j = i - 1;
-        while (j >= 0 && ang[t] < ang[idx[j]]) {
+        while (j >= 0 && before(t, idx[j])) {
             idx[j + 1] = idx[j];
             j--;
         }
@@ -184,7 +216,9 @@ int prologue(const double* clon, const double* clat, const double* hsurf, const 
         return -1;
     }
     // --- pass 1: triangles (and added vertices) per ICON vertex -------------
-    std::vector<int> ntri_v(nvert), nadd_v(nvert);
+    HostVec<int>& ntri_v = out.vcount;  // triangles of vertex v, then added vertices
+    ntri_v.resize((size_t)2 * nvert);
+    int* nadd_v = ntri_v.data() + nvert;
     parallel_for(nvert, [&](long long b, long long e) {
         for (long long v = b; v < e; v++) {
             int n = 0;
@@ -199,7 +233,9 @@ int prologue(const double* clon, const double* clat, const double* hsurf, const 
             }
         }
     });
-    std::vector<long long> toff(nvert + 1), aoff(nvert + 1);
+    HostVec<long long>& toff = out.voff;  // prefix sums of both counts
+    toff.resize((size_t)2 * (nvert + 1));
+    long long* aoff = toff.data() + (nvert + 1);
     toff[0] = aoff[0] = 0;
     for (int v = 0; v < nvert; v++) {
         toff[v + 1] = toff[v] + ntri_v[v];
@@ -212,7 +248,8 @@ int prologue(const double* clon, const double* clat, const double* hsurf, const 
         return -1;
     }
     // mesh vertices as lon, lat, elevation (reference :234-238, :300-301)
-    HostVec<double> P((size_t)nmv * 3);  // every entry is written below
+    HostVec<double>& P = out.work;  // every entry is written below
+    P.resize((size_t)nmv * 3);
     out.tri.resize((size_t)ntri * 3);
     parallel_for(ncell, [&](long long b, long long e) {
         for (long long i = b; i < e; i++) {

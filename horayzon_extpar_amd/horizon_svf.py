@@ -13,8 +13,8 @@ Differences, all of them supersets:
   printing and returning garbage;
 * keyword-only ``search_mode``, ``leaf_size``, ``block_size``,
   ``collect_counters``, ``trace_kernel``, ``refill``, ``bvh_morton``,
-  ``bvh_width``, ``prologue``, ``whole_cells``, ``certify`` and ``stats``
-  expose the library options (``hz_options`` in include/horayzon.h).  The
+  ``bvh_width``, ``prologue``, ``whole_cells``, ``certify``, ``cache`` and
+  ``stats`` expose the library options (``hz_options`` in include/horayzon.h).  The
   default ``search_mode=0`` is the reference walk; the lattice searches
   (1 climb, 2 learned start) equal it with the default ``certify=1``.
 """
@@ -105,7 +105,7 @@ def horizon_svf_comp_py(clon, clat, hsurf, vlon, vlat, cells_of_vertex, num_hori
                         search_mode=None, leaf_size=None, block_size=None,
                         collect_counters=None, trace_kernel=None, refill=None,
                         bvh_morton=None, bvh_width=None, prologue=None, whole_cells=None,
-                        certify=None, return_slope_aspect=False, stats=None):
+                        certify=None, cache=None, return_slope_aspect=False, stats=None):
     """Compute the terrain horizon [deg] and sky view factor [-].
 
     Same contract as the reference binding (horizon_svf.pyx:15-124):
@@ -130,7 +130,7 @@ def horizon_svf_comp_py(clon, clat, hsurf, vlon, vlat, cells_of_vertex, num_hori
                           block_size=block_size, collect_counters=collect_counters,
                           trace_kernel=trace_kernel, refill=refill, bvh_morton=bvh_morton,
                           bvh_width=bvh_width, prologue=prologue, whole_cells=whole_cells,
-                          certify=certify)
+                          certify=certify, cache=cache)
     if return_slope_aspect:
         with ResidentDomain(clon, clat, hsurf, vlon, vlat, cells_of_vertex, grid_type,
                             ray_org_elev, prologue=opt.prologue) as dom:

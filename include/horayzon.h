@@ -73,7 +73,14 @@ typedef struct hz_options {
                              differs from the walk where its rays pass through mesh holes.
                              Needs (cell, azimuth) work items (persistent kernel, whole_cells
                              0); without memory for them the walk runs everywhere */
-    int reserved[2];
+    int cache;            /* one-shot calls (horizon_svf_comp_ex with the host prologue): 1
+                             (default) = keep the device buffers, streams and pinned staging
+                             of the call for the next call on the device whose grid has the
+                             same sizes (cells, triangles, mesh vertices), which then only
+                             uploads its scene; a call with other sizes frees them first.
+                             0 = free everything at the end of the call.  hz_cache_release
+                             frees what is kept.  The entry points without options use 1 */
+    int reserved[1];
 } hz_options;
 
 typedef struct hz_stats {
@@ -152,6 +159,14 @@ void horizon_svf_comp_fortran(double* clon, double* clat, double* hsurf, double*
                               double* skyview, int num_cell, int num_vertex, int num_hori,
                               int grid_type, double dist_search, double ray_org_elev,
                               int refine_factor, int svf_type, char* buffer, int* buffer_len);
+
+/*
+ * Frees the device buffers the one-shot calls keep for the next call of the
+ * same grid size (option cache).  No counterpart in the reference, which
+ * builds and frees its scene in every call (mo_lradtopo_horayzon.cpp:395-432,
+ * :780-785).
+ */
+void hz_cache_release(void);
 
 /* Message of the calling thread's last failure ("" if none). */
 const char* hz_last_error(void);
