@@ -52,6 +52,8 @@ def main():
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the batched 64-domain leg (config 4)")
+    ap.add_argument("--no-e2e", action="store_true",
+                    help="skip the end-to-end C-ABI call legs")
     ap.add_argument("--weights", default="")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="first run one device-to-device copy of MIB MiB (a plain copy "
@@ -145,9 +147,13 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
                      "traffic": traffic[0] if traffic else None,
-                     "traffic_source": traffic[1] if traffic else None,
+                     "traffic_gather_scaled": traffic[1] if traffic else None,
+                     "traffic_source": traffic[2] if traffic else None,
                      "frac_measured_traffic": (traffic[0] / (tr * 1e-3) / 1e9 / PEAK_HBM_GBS
                                                if traffic else None),
+                     "frac_measured_traffic_gather_scaled": (
+                         traffic[1] / (tr * 1e-3) / 1e9 / PEAK_HBM_GBS
+                         if traffic and traffic[1] else None),
                      "limiter": "instruction issue under lane divergence (no unit "
                                 "saturated: VALU busy with a minority of lanes active, "
                                 "HBM below peak); the HBM roofline is the bound the "
@@ -163,8 +169,9 @@ def main():
         out["learned"] = learned_out
         out["learned_vs_plain_speedup"] = learned_out["speedup_vs_plain"]
         out["learned_vs_climb_speedup"] = learned_out["learned_vs_climb_speedup"]
-    out["end_to_end"] = end_to_end_leg(dom_in, opts)
-    out["end_to_end_device_prologue"] = end_to_end_leg(dom_in, dict(opts, prologue="device"))
+    if not args.no_e2e:
+        out["end_to_end"] = end_to_end_leg(dom_in, opts)
+        out["end_to_end_device_prologue"] = end_to_end_leg(dom_in, dict(opts, prologue="device"))
     if not args.no_batch and mode == 0:
         out["batched_c4"] = batch_leg(args)
     if not args.no_cpu:
@@ -368,21 +375,29 @@ def calibrate_gather(mib):
 
 
 def profiled_traffic(args):
-    """(HBM bytes per launch, source) of the default walk kernel on C2, from
-    the committed rocprofv3 PMC summary (FETCH_SIZE and WRITE_SIZE, scaled by
-    a 1 GiB copy); None for any other configuration."""
+    """(HBM bytes per launch on the copy calibration, on the gather
+    calibration, source) of the default walk kernel on C2, from the
+    committed rocprofv3 PMC summary (FETCH_SIZE and WRITE_SIZE, FETCH scaled
+    by a streaming 1 GiB copy or by a random 64-byte gather of 1 GiB); None
+    for any other configuration."""
     default = (args.config == "c2" and args.mode == "walk" and args.kernel == 0
                and args.width in (0, 2) and args.leaf in (0, 1) and args.block in (0, 128)
                and args.morton == 0)
     path = os.path.join(ROOT, "profiles", "k_trace_c2_pmc.md")
     if not default or not os.path.exists(path):
         return None
+    copy = gather = None
     for line in open(path):
         if line.startswith("traffic_bytes_per_launch:"):
-            return (int(line.split()[1]),
-                    "profiles/k_trace_c2_pmc.md: rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 "
-                    "(1 GiB copy calibration), " + line.split(None, 2)[2].strip())
-    return None
+            copy = int(line.split()[1])
+        elif line.startswith("traffic_bytes_per_launch_gather_scaled:"):
+            gather = int(line.split()[1])
+    if copy is None:
+        return None
+    return (copy, gather,
+            "profiles/k_trace_c2_pmc.md: rocprofv3 FETCH_SIZE (x 2.0 on a 1 GiB streaming "
+            "copy; x 0.563 on a random 64-byte gather) + WRITE_SIZE x 1.0, "
+            "k_trace_pt<0, false, 128, 0, false, false>")
 
 
 def accuracy_leg(dom_in, h, s, h_orc, s_orc, threads, nsample=48):

@@ -232,7 +232,10 @@ __device__ __forceinline__ int trav_step(const BvhNode* __restrict__ nodes,
                                          unsigned& ntest, int& err, int& hit_leaf) {
     const float4* np = reinterpret_cast<const float4*>(nodes + (node & kLeafFirstMask));
     const float4 a = np[0], b = np[1], c = np[2], d = np[3];
-    float4 e4 = make_float4(0.f, 0.f, 0.f, 0.f), f4 = e4, g4 = e4, h4 = e4;
+    // the second half is read only for nodes with inline triangles, and only
+    // their leaf children test it: elsewhere its value is never used, so it
+    // is left unset rather than cleared on every step
+    float4 e4, f4, g4, h4;
     if (node & kWideBit) {  // inline triangle(s): second half of the node
         e4 = np[4];
         f4 = np[5];
@@ -328,7 +331,10 @@ __device__ __forceinline__ int trav_step4(const Bvh4Node* __restrict__ nodes,
                                           unsigned& ntest, int& err, int& hit_leaf) {
     const float4* np = reinterpret_cast<const float4*>(nodes + (node & kLeafFirstMask));
     const float4 a = np[0], b = np[1], c = np[2], d = np[3];
-    float4 e4 = make_float4(0.f, 0.f, 0.f, 0.f), f4 = e4, g4 = e4, h4 = e4;
+    // the second half is read only for nodes with inline triangles, and only
+    // their leaf children test it: elsewhere its value is never used, so it
+    // is left unset rather than cleared on every step
+    float4 e4, f4, g4, h4;
     if (node & kWideBit) {  // inline triangle(s): second half of the node
         e4 = np[4];
         f4 = np[5];
