@@ -6,8 +6,14 @@ per-cell descriptor computed on the device (24 azimuths x 8 log-spaced rings
 of maximum terrain elevation angle, 0.25 .. 40 km, BF16); its output is the
 horizon per azimuth in degrees.  ``search_mode=2`` starts each azimuth's
 lattice search at the predicted elevation: the search climbs while the query
-hits and descends while it misses, so a wrong prediction costs queries, not
-accuracy; the plain search stays the fallback and decides every result.
+hits and descends while it misses, so where the hits along the azimuth are
+monotone in elevation a wrong prediction costs queries, not accuracy.  Where
+a ray can pass through a hole of the mesh they need not be, and there the
+lattice search from any start can differ from the reference walk; with the
+default ``certify=1`` the library runs the reference walk itself for every
+cell with a mesh-boundary point within reach, so the learned search's
+results equal the walk's everywhere (to the walk's FP64 rounding of its
+carried elevation, ~1e-14 deg).
 
 Training targets are horizons the method itself produces (the clip-aware
 lattice search, ``search_mode=1``) on synthetic terrains from fresh seeds.
