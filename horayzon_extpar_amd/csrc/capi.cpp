@@ -154,6 +154,11 @@ struct hz_domain {
     hipEvent_t pev[3] = {nullptr, nullptr, nullptr};  // fork, predictor start, join
     bool ran = false;
     bool host_scene = false;  // built from a host prologue (the one-shot calls' cache holds these)
+    // a kept domain reloaded by a one-shot call builds its default tree while
+    // the host computes the cell frames; the run then skips the build
+    bool bvh_pre = false;
+    float bvh_pre_ms = 0.0f;
+    hipEvent_t pre_ev[2] = {nullptr, nullptr};
 };
 
 // The progress report the reference prints to stdout (mo_lradtopo_horayzon.cpp
@@ -252,6 +257,8 @@ void hz_domain_destroy(hz_domain* d) {
     for (auto& e : d->bev)
         if (e) (void)hipEventDestroy(e);
     for (auto& e : d->cev)
+        if (e) (void)hipEventDestroy(e);
+    for (auto& e : d->pre_ev)
         if (e) (void)hipEventDestroy(e);
     if (d->xstream) (void)hipStreamDestroy(d->xstream);
     if (d->ystream) (void)hipStreamDestroy(d->ystream);
@@ -712,9 +719,15 @@ int hz_domain_run(hz_domain* d, int num_hori, double dist_search, int refine_fac
     }
     d->bvh.nodes = d->d_nodes;
     d->bvh.tris = d->d_tris;
-    rc = lbvh_build(d->work, d->d_vert, d->d_tri, (int)d->ntri, leaf, o.bvh_morton, width, st,
-                    d->bvh, o.collect_counters != 0, err);
-    if (rc) return fail(rc == -3 ? HZ_ENOMEM : (rc == -1 ? HZ_EINVAL : HZ_EDEVICE), err);
+    // the tree of the default options may already be built (one-shot reload)
+    const bool use_pre = d->bvh_pre && leaf == 1 && o.bvh_morton == 0 && width == 2 &&
+                         o.collect_counters == 0;
+    d->bvh_pre = false;
+    if (!use_pre) {
+        rc = lbvh_build(d->work, d->d_vert, d->d_tri, (int)d->ntri, leaf, o.bvh_morton, width,
+                        st, d->bvh, o.collect_counters != 0, err);
+        if (rc) return fail(rc == -3 ? HZ_ENOMEM : (rc == -1 ? HZ_EINVAL : HZ_EDEVICE), err);
+    }
     RCHK(hipEventRecord(d->ev[1], st));
     const int* order = nullptr;
     if (o.cell_order) {
@@ -849,6 +862,7 @@ int hz_domain_run(hz_domain* d, int num_hori, double dist_search, int refine_fac
     }
     float ms_bvh = 0, ms_ord = 0, ms_trace = 0;
     RCHK(hipEventElapsedTime(&ms_bvh, d->ev[0], d->ev[1]));
+    if (use_pre) ms_bvh += d->bvh_pre_ms;
     RCHK(hipEventElapsedTime(&ms_ord, d->ev[1], d->ev[2]));
     float ms_pred = 0;  // the predictor stage on its own stream
     if (learned) RCHK(hipEventElapsedTime(&ms_pred, d->pev[1], d->pev[2]));
@@ -1033,11 +1047,14 @@ void cache_put(hz_domain* d) {
     hz_domain_destroy(old);
 }
 
-// Load a host scene of the same sizes into a kept domain: the arrays are
-// uploaded, and what was derived from the previous scene (certificate index,
-// slope adjacency, predictor, cell flags, run state) is dropped.
-int domain_reload(hz_domain* d, const HostScene& hs, const double* hsurf, int num_cell,
-                  int grid_type, hz_stats* stats, double t0, double t1) {
+// Load a host scene of the same sizes into a kept domain: upload the FP32
+// scene, start the default tree build on the device, compute the cell frames
+// on the host meanwhile (prologue_frames), then upload them.  What was
+// derived from the previous scene (certificate index, slope adjacency,
+// predictor, cell flags, run state) is dropped.
+int domain_reload(hz_domain* d, HostScene& hs, const double* hsurf, const int* cov,
+                  int num_cell, int num_vertex, int grid_type, double ray_org_elev,
+                  hz_stats* stats, double t0, double t1) {
     hipError_t e = hipSetDevice(d->device);
     d->ran = false;
     d->last_nhori = 0;
@@ -1045,6 +1062,7 @@ int domain_reload(hz_domain* d, const HostScene& hs, const double* hsurf, int nu
     d->near_flags = false;
     d->predictor = nullptr;
     d->have_pred = false;
+    d->bvh_pre = false;
     if (e == hipSuccess && d->d_adj_off) e = hipFree(d->d_adj_off);
     if (e == hipSuccess && d->d_adj) e = hipFree(d->d_adj);
     d->d_adj_off = nullptr;
@@ -1059,23 +1077,40 @@ int domain_reload(hz_domain* d, const HostScene& hs, const double* hsurf, int nu
     d->lat0 = hs.lat0;
     std::memcpy(d->bounds_xy, hs.bounds_xy, sizeof(d->bounds_xy));
     const size_t nv = hs.vert32.size(), nt = hs.tri.size();
+    for (auto& x : d->pre_ev)
+        if (e == hipSuccess && !x) e = hipEventCreate(&x);
     if (e == hipSuccess && nv)
         e = d->stager.copy(d->d_vert, hs.vert32.data(), nv * sizeof(float), true, d->stream);
     if (e == hipSuccess && nt)
         e = d->stager.copy(d->d_tri, hs.tri.data(), nt * sizeof(int), true, d->stream);
+    std::string err;
+    if (e == hipSuccess) e = hipEventRecord(d->pre_ev[0], d->stream);
+    if (e == hipSuccess) {
+        d->bvh.nodes = d->d_nodes;
+        d->bvh.tris = d->d_tris;
+        const int rc = lbvh_build(d->work, d->d_vert, d->d_tri, (int)d->ntri, 1, 0, 2, d->stream,
+                                  d->bvh, false, err);
+        if (rc) return fail(rc == -3 ? HZ_ENOMEM : HZ_EDEVICE, err);
+        e = hipEventRecord(d->pre_ev[1], d->stream);
+    }
+    const double tf0 = now_ms();
+    prologue_frames(cov, num_cell, num_vertex, ray_org_elev, hs);
+    const double tf1 = now_ms();
     if (e == hipSuccess)
         e = d->stager.copy(d->d_cells, hs.frame.data(), (size_t)num_cell * sizeof(CellFrame), true,
                            d->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(d->stream);
+    if (e == hipSuccess) e = hipEventElapsedTime(&d->bvh_pre_ms, d->pre_ev[0], d->pre_ev[1]);
     if (e != hipSuccess)
         return fail(e == hipErrorOutOfMemory ? HZ_ENOMEM : HZ_EDEVICE,
                     std::string("reloading a kept domain: ") + hipGetErrorString(e));
+    d->bvh_pre = true;
     if (stats) {
         std::memset(stats, 0, sizeof(*stats));
         stats->num_triangles = d->ntri;
         stats->num_mesh_vertices = d->nmeshvert;
-        stats->t_prologue_ms = t1 - t0;
-        stats->t_upload_ms = now_ms() - t1;
+        stats->t_prologue_ms = (t1 - t0) + (tf1 - tf0);
+        stats->t_upload_ms = (now_ms() - t1) - (tf1 - tf0);
         stats->lon0 = d->lon0;
         stats->lat0 = d->lat0;
     }
@@ -1142,8 +1177,8 @@ int horizon_svf_comp_ex(const double* clon, const double* clat, const double* hs
         std::string err;
         const double tp0 = now_ms();
         if (!rc) {
-            rc = prologue(clon, clat, hsurf, vlon, vlat, cov, num_cell, num_vertex, grid_type,
-                          ray_org_elev, hs, err);
+            rc = prologue_scene(clon, clat, hsurf, vlon, vlat, cov, num_cell, num_vertex,
+                                grid_type, hs, err);
             if (rc) rc = fail(HZ_EINVAL, err);
         }
         if (!rc) {
@@ -1152,12 +1187,14 @@ int horizon_svf_comp_ex(const double* clon, const double* clat, const double* hs
             if (hipGetDevice(&dev) != hipSuccess) dev = -1;
             d = cache_take(dev, num_cell, (long long)hs.tri.size() / 3, hs.num_mesh_vertices);
             if (d) {
-                rc = domain_reload(d, hs, hsurf, num_cell, grid_type, s, tp0, tp1);
+                rc = domain_reload(d, hs, hsurf, cov, num_cell, num_vertex, grid_type,
+                                   ray_org_elev, s, tp0, tp1);
                 if (rc) {
                     hz_domain_destroy(d);
                     d = nullptr;
                 }
             } else {
+                prologue_frames(cov, num_cell, num_vertex, ray_org_elev, hs);
                 rc = make_domain(clon, clat, hsurf, vlon, vlat, cov, num_cell, num_vertex,
                                  grid_type, ray_org_elev, HZ_PROLOGUE_HOST, hs, tp0, &d, s);
             }

@@ -205,11 +205,23 @@ struct HostScene {
     HostVec<double> work;   // FP64 mesh vertices (lon/lat/h, then ECEF, then ENU)
     HostVec<int> vcount;    // per ICON vertex: triangles, added mesh vertices
     HostVec<long long> voff;  // their prefix sums
+    double centre[3] = {0.0, 0.0, 0.0};  // earth centre and north pole in ENU
+    double pole[3] = {0.0, 0.0, 0.0};
 };
 
 int prologue(const double* clon, const double* clat, const double* hsurf, const double* vlon,
              const double* vlat, const int* cells_of_vertex, int num_cell, int num_vertex,
              int grid_type, double ray_org_elev, HostScene& out, std::string& err);
+// The two halves of prologue(): the mesh, ECEF and ENU transforms and the FP32
+// scene (vert32, tri, bounds, ENU origin), then the per-cell frames and hole
+// flags (frame), which read the FP64 ENU cell points prologue_scene leaves in
+// out.work.  A caller can start the tree build between the two.
+int prologue_scene(const double* clon, const double* clat, const double* hsurf,
+                   const double* vlon, const double* vlat, const int* cells_of_vertex,
+                   int num_cell, int num_vertex, int grid_type, HostScene& out,
+                   std::string& err);
+void prologue_frames(const int* cells_of_vertex, int num_cell, int num_vertex,
+                     double ray_org_elev, HostScene& out);
 
 WalkParams make_walk_params(int num_hori, int refine_factor, int svf_type, double dist_search,
                             int ncell);

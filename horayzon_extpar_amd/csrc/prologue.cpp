@@ -211,6 +211,16 @@ void enu_frame_host(const double* clon, const double* clat, const double* hsurf,
 int prologue(const double* clon, const double* clat, const double* hsurf, const double* vlon,
              const double* vlat, const int* cov, int ncell, int nvert, int grid_type,
              double ray_org_elev, HostScene& out, std::string& err) {
+    const int rc = prologue_scene(clon, clat, hsurf, vlon, vlat, cov, ncell, nvert, grid_type,
+                                  out, err);
+    if (rc) return rc;
+    prologue_frames(cov, ncell, nvert, ray_org_elev, out);
+    return 0;
+}
+
+int prologue_scene(const double* clon, const double* clat, const double* hsurf,
+                   const double* vlon, const double* vlat, const int* cov, int ncell, int nvert,
+                   int grid_type, HostScene& out, std::string& err) {
     if (ncell < 1 || nvert < 0) {
         err = "num_cell must be >= 1 and num_vertex >= 0";
         return -1;
@@ -311,8 +321,8 @@ int prologue(const double* clon, const double* clat, const double* hsurf, const 
         r[1] = -sa * co * dx - sa * so * dy + ca * dz;
         r[2] = ca * co * dx + ca * so * dy + sa * dz;
     };
-    const double* centre = E.centre;
-    const double* pole = E.pole;
+    std::memcpy(out.centre, E.centre, sizeof(out.centre));
+    std::memcpy(out.pole, E.pole, sizeof(out.pole));
     out.vert32.resize((size_t)nmv * 3);
     std::mutex bmu;
     float bx[4] = {3e38f, 3e38f, -3e38f, -3e38f};
@@ -342,6 +352,15 @@ int prologue(const double* clon, const double* clat, const double* hsurf, const 
     });
     if (nmv == 0) bx[0] = bx[1] = bx[2] = bx[3] = 0.0f;
     std::memcpy(out.bounds_xy, bx, sizeof(bx));
+    out.num_mesh_vertices = nmv;
+    return 0;
+}
+
+void prologue_frames(const int* cov, int ncell, int nvert, double ray_org_elev,
+                     HostScene& out) {
+    const double* P = out.work.data();
+    const double* centre = out.centre;
+    const double* pole = out.pole;
     // --- per-cell frame (reference :693-722) ---------------------------------
     out.frame.resize(ncell);
     parallel_for(ncell, [&](long long b, long long e) {
@@ -384,8 +403,6 @@ int prologue(const double* clon, const double* clat, const double* hsurf, const 
             if (c >= 0 && c < ncell) out.frame[c].pad |= 1;
         }
     }
-    out.num_mesh_vertices = nmv;
-    return 0;
 }
 
 void cell_adjacency(const int* tri, long long ntri, int ncell, std::vector<int>& off,
