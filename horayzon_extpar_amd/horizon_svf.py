@@ -76,8 +76,11 @@ def validate(clon, clat, hsurf, vlon, vlat, cells_of_vertex, num_hori, grid_type
         raise ValueError("Inconsistent lengths of input arrays 'vlon' and 'vlat'")
     if cells_of_vertex.shape[0] != 6:
         raise ValueError("First dimension of 'cells_of_vertex' must have length 6")
-    if not np.all((cells_of_vertex >= 0) & (cells_of_vertex <= clon.size - 1)
-                  | (cells_of_vertex == -2)):
+    # every entry in [0, ncell - 1] or -2 (the same test as the reference's
+    # elementwise one, as three reductions)
+    if cells_of_vertex.size and not (cells_of_vertex.min() >= -2
+                                     and cells_of_vertex.max() <= clon.size - 1
+                                     and not (cells_of_vertex == -1).any()):
         raise ValueError("Indices of 'cells_of_vertex' out of range")
     if (num_hori < 4) or (num_hori > 1440):
         raise ValueError("'num_hori' must be in the range [4, 1440]")
