@@ -35,7 +35,7 @@ class HzOptions(ctypes.Structure):
                 ("bvh_morton", ctypes.c_int), ("queue_chunk", ctypes.c_int),
                 ("bvh_width", ctypes.c_int), ("prologue", ctypes.c_int),
                 ("whole_cells", ctypes.c_int), ("certify", ctypes.c_int),
-                ("cache", ctypes.c_int), ("reserved", ctypes.c_int * 1)]
+                ("cache", ctypes.c_int), ("bvh_sah", ctypes.c_int)]
 
 
 class HzStats(ctypes.Structure):

@@ -616,6 +616,8 @@ int hz_domain_run(hz_domain* d, int num_hori, double dist_search, int refine_fac
     if (refill < 1 || refill > 64) return fail(HZ_EINVAL, "refill must be in [1, 64]");
     if (o.trace_kernel == 1) width = 2;  // the lane-per-cell kernel walks binary trees
     if (o.certify < 0 || o.certify > 1) return fail(HZ_EINVAL, "certify must be 0 or 1");
+    if (o.bvh_sah < 0 || o.bvh_sah > 1) return fail(HZ_EINVAL, "bvh_sah must be 0 or 1");
+    if (o.bvh_sah && width != 2) return fail(HZ_EINVAL, "bvh_sah needs the binary tree (bvh_width 2)");
     const bool certify = o.certify == 1 && o.search_mode != HZ_MODE_WALK;
     if (certify && (o.trace_kernel == 1 || o.whole_cells))
         return fail(HZ_EINVAL,
@@ -721,11 +723,11 @@ int hz_domain_run(hz_domain* d, int num_hori, double dist_search, int refine_fac
     d->bvh.tris = d->d_tris;
     // the tree of the default options may already be built (one-shot reload)
     const bool use_pre = d->bvh_pre && leaf == 1 && o.bvh_morton == 0 && width == 2 &&
-                         o.collect_counters == 0;
+                         o.collect_counters == 0 && o.bvh_sah == 0;
     d->bvh_pre = false;
     if (!use_pre) {
         rc = lbvh_build(d->work, d->d_vert, d->d_tri, (int)d->ntri, leaf, o.bvh_morton, width,
-                        st, d->bvh, o.collect_counters != 0, err);
+                        st, d->bvh, o.collect_counters != 0, err, o.bvh_sah == 1);
         if (rc) return fail(rc == -3 ? HZ_ENOMEM : (rc == -1 ? HZ_EINVAL : HZ_EDEVICE), err);
     }
     RCHK(hipEventRecord(d->ev[1], st));
@@ -932,8 +934,9 @@ int hz_domain_bvh(hz_domain* d, const hz_options* opt, void* nodes, long long* n
     std::string err;
     d->bvh.nodes = d->d_nodes;
     d->bvh.tris = d->d_tris;
+    if (o.bvh_sah && width != 2) return fail(HZ_EINVAL, "bvh_sah needs the binary tree (bvh_width 2)");
     int rc = lbvh_build(d->work, d->d_vert, d->d_tri, (int)d->ntri, leaf, o.bvh_morton, width,
-                        d->stream, d->bvh, false, err);
+                        d->stream, d->bvh, false, err, o.bvh_sah == 1);
     if (rc) return fail(rc == -3 ? HZ_ENOMEM : (rc == -1 ? HZ_EINVAL : HZ_EDEVICE), err);
     long long count = 0;
     if (d->ntri > 0) {
@@ -1396,7 +1399,7 @@ int hz_batch_run(hz_batch* b, int num_hori, double dist_search, int refine_facto
         d->bvh.nodes = d->d_nodes;
         d->bvh.tris = d->d_tris;
         rc = lbvh_build(d->work, d->d_vert, d->d_tri, (int)d->ntri, leaf, o.bvh_morton, width, st,
-                        d->bvh, false, err);
+                        d->bvh, false, err, o.bvh_sah == 1 && width == 2);
         if (rc) return fail(rc == -3 ? HZ_ENOMEM : (rc == -1 ? HZ_EINVAL : HZ_EDEVICE), err);
         if (o.cell_order) {
             rc = order_cells(d->work, d->d_cells, d->ncell, d->d_order, st, err);

@@ -323,9 +323,13 @@ size_t lbvh_bind(LbvhWork& w, int n, char* base, std::string& err);
 // morton_mode 0: 3D codes in a cube of the largest extent, 1: 2D x/y codes;
 // width 2: binary BvhNode tree, 4: four-wide Bvh4Node tree (its node count
 // is left on the device in *out.d_num_nodes and num_nodes is set to -1)
+// sah_top (binary trees of 4096 or more triangles): the top of the Morton
+// tree (nodes over more than max(leaf_size, 8, ntri / 8192) triangles) is
+// rebuilt by a binned SAH over the subtrees below it; synchronous on stream.
+// The depth of want_depth is then that of the Morton tree.
 int lbvh_build(LbvhWork& w, const float* vert, const int* tri, int ntri, int leaf_size,
                int morton_mode, int width, hipStream_t stream, DeviceBvh& out, bool want_depth,
-               std::string& err);
+               std::string& err, bool sah_top = false);
 
 // Order of the cells: those next to a mesh hole first (CellFrame flag bit 0),
 // each group in Morton order of the ray origins (2D, x/y of the ENU frame).

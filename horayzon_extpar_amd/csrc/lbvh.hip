@@ -25,7 +25,10 @@
 // occlusion answer is then the OR of the triangle test over all triangles,
 // whatever the tree looks like.
 
+#include <algorithm>
 #include <cstring>
+#include <string>
+#include <vector>
 
 #include <rocprim/rocprim.hpp>
 
@@ -668,6 +671,283 @@ inline int grid(int n) { return (n + kBlock - 1) / kBlock; }
 
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// SAH rebuild of the top of the tree (option): the interior nodes whose leaf
+// range holds more than T triangles form a connected top part of the Morton
+// tree; the subtrees hanging below it (and leaves hanging from it) are cut
+// off as K items, the top is rebuilt over the items by a binned surface-area
+// heuristic on the host (K ≈ 8k, microseconds of work), and the K - 1 new
+// nodes are written into the slots of the K - 1 old top nodes, the root into
+// slot 0.  Item boxes are the grown boxes their old parents stored; a new
+// node's box is the union of grown boxes, a superset as the traversal needs.
+// The answer of a query stays the OR over all triangles, so results do not
+// change, only which nodes a ray visits.
+// ---------------------------------------------------------------------------
+
+struct SahItem {
+    float lo[3], hi[3];
+    int ref;
+    int pad;
+    float tri[9];  // the inline triangle of a single-triangle leaf
+    float pad2[3];
+};
+static_assert(sizeof(SahItem) == 80, "SahItem layout");
+
+namespace {
+
+__global__ __launch_bounds__(kBlock) void k_sah_cut(const int* __restrict__ range,
+                                                   const int* __restrict__ child, int n, int T,
+                                                   const BvhNode* __restrict__ nodes,
+                                                   SahItem* __restrict__ items,
+                                                   int* __restrict__ top,
+                                                   unsigned* __restrict__ counts, int cap) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n - 1) return;
+    if (range[2 * i + 1] - range[2 * i] + 1 <= T) return;
+    const unsigned ti = atomicAdd(&counts[0], 1u);
+    if ((int)ti < cap) top[ti] = i;
+    const BvhNode& nd = nodes[i];
+    for (int s = 0; s < 2; s++) {
+        const int c = child[2 * i + s];
+        if (c >= 0 && range[2 * c + 1] - range[2 * c] + 1 > T) continue;  // a top node
+        const unsigned k = atomicAdd(&counts[1], 1u);
+        if ((int)k >= cap) continue;
+        SahItem it;
+        it.lo[0] = s ? nd.lx1 : nd.lx0;
+        it.hi[0] = s ? nd.hx1 : nd.hx0;
+        it.lo[1] = s ? nd.ly1 : nd.ly0;
+        it.hi[1] = s ? nd.hy1 : nd.hy0;
+        it.lo[2] = s ? nd.lz1 : nd.lz0;
+        it.hi[2] = s ? nd.hz1 : nd.hz0;
+        it.ref = s ? nd.r1 : nd.r0;
+        it.pad = 0;
+        for (int q = 0; q < 9; q++) it.tri[q] = nd.tri[9 * s + q];
+        it.pad2[0] = it.pad2[1] = it.pad2[2] = 0.f;
+        items[k] = it;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_sah_scatter(const BvhNode* __restrict__ src,
+                                                       const int* __restrict__ slot, int m,
+                                                       BvhNode* __restrict__ nodes) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < m) nodes[slot[i]] = src[i];
+}
+
+// Binned SAH build over the items on the host.  Node k of `out` has children
+// a / b: >= 0 another new node, < 0 item ~x.
+struct SahBuild {
+    const std::vector<SahItem>& it;
+    std::vector<int2> kids;
+    std::vector<float> box;  // 6 floats per new node: lo xyz, hi xyz
+    std::vector<char> wide;  // the node holds an inline triangle
+
+    explicit SahBuild(const std::vector<SahItem>& items) : it(items) {}
+
+    static float area(const float* b) {
+        const float dx = std::max(0.f, b[3] - b[0]), dy = std::max(0.f, b[4] - b[1]),
+                    dz = std::max(0.f, b[5] - b[2]);
+        return dx * dy + dy * dz + dz * dx;
+    }
+    static void grow(float* b, const float* lo, const float* hi) {
+        for (int a = 0; a < 3; a++) {
+            b[a] = std::min(b[a], lo[a]);
+            b[3 + a] = std::max(b[3 + a], hi[a]);
+        }
+    }
+    static void empty(float* b) {
+        b[0] = b[1] = b[2] = 3e38f;
+        b[3] = b[4] = b[5] = -3e38f;
+    }
+    // build over idx[lo, hi); returns the child reference (node or ~item)
+    int build(std::vector<int>& idx, int lo, int hi) {
+        if (hi - lo == 1) return ~idx[lo];
+        float cb[6];
+        empty(cb);
+        for (int k = lo; k < hi; k++) {
+            const SahItem& x = it[idx[k]];
+            float c[3];
+            for (int a = 0; a < 3; a++) c[a] = 0.5f * (x.lo[a] + x.hi[a]);
+            grow(cb, c, c);
+        }
+        int axis = 0;
+        for (int a = 1; a < 3; a++)
+            if (cb[3 + a] - cb[a] > cb[3 + axis] - cb[axis]) axis = a;
+        const float ext = cb[3 + axis] - cb[axis];
+        int mid = lo + (hi - lo) / 2;
+        if (ext > 0.f) {
+            constexpr int kBins = 16;
+            float bb[kBins][6];
+            int bc[kBins] = {};
+            for (auto& b : bb) empty(b);
+            auto bin = [&](const SahItem& x) {
+                const float c = 0.5f * (x.lo[axis] + x.hi[axis]);
+                int k = (int)((c - cb[axis]) / ext * kBins);
+                return std::min(kBins - 1, std::max(0, k));
+            };
+            for (int k = lo; k < hi; k++) {
+                const SahItem& x = it[idx[k]];
+                const int b = bin(x);
+                grow(bb[b], x.lo, x.hi);
+                bc[b]++;
+            }
+            float best = 3e38f;
+            int split = -1;
+            float lb[6];
+            empty(lb);
+            int ln = 0;
+            for (int s = 0; s < kBins - 1; s++) {
+                grow(lb, bb[s], bb[s] + 3);
+                ln += bc[s];
+                float rb[6];
+                empty(rb);
+                int rn = 0;
+                for (int t = s + 1; t < kBins; t++) {
+                    grow(rb, bb[t], bb[t] + 3);
+                    rn += bc[t];
+                }
+                if (ln == 0 || rn == 0) continue;
+                const float cost = area(lb) * ln + area(rb) * rn;
+                if (cost < best) {
+                    best = cost;
+                    split = s;
+                }
+            }
+            if (split >= 0) {
+                const auto it_mid = std::partition(
+                    idx.begin() + lo, idx.begin() + hi,
+                    [&](int q) { return bin(it[q]) <= split; });
+                mid = (int)(it_mid - idx.begin());
+            } else {
+                std::sort(idx.begin() + lo, idx.begin() + hi, [&](int p, int q) {
+                    return it[p].lo[axis] + it[p].hi[axis] < it[q].lo[axis] + it[q].hi[axis];
+                });
+            }
+        }
+        const int me = (int)kids.size();
+        kids.push_back(make_int2(0, 0));
+        box.resize(box.size() + 6);
+        wide.push_back(0);
+        const int a = build(idx, lo, mid);
+        const int b = build(idx, mid, hi);
+        kids[me] = make_int2(a, b);
+        float* mb = &box[(size_t)me * 6];
+        empty(mb);
+        for (int r : {a, b}) {
+            if (r >= 0) {
+                const float* cbx = &box[(size_t)r * 6];
+                grow(mb, cbx, cbx + 3);
+            } else {
+                const SahItem& x = it[~r];
+                grow(mb, x.lo, x.hi);
+                const int code = ~x.ref;
+                if (x.ref < 0 && (code >> kLeafFirstBits) == 0) wide[me] = 1;
+            }
+        }
+        return me;
+    }
+};
+
+}  // namespace
+
+// Rebuilds the top of the binary tree in out.nodes (after k_pack) with the
+// SAH; w holds the Karras arrays of the build.  Synchronous on st.
+static int lbvh_sah_top(LbvhWork& w, int n, int leaf_size, hipStream_t st, DeviceBvh& out,
+                        std::string& err) {
+    // trees of fewer than 4096 triangles keep the Morton top
+    if (n < 4096) return 0;
+    const int T = std::max(std::max(leaf_size, 8), n / 8192);
+    const int cap = 2 * (n / T) + 16;  // items, and top nodes (one fewer)
+    // scratch in the build's range-union area (96 n bytes or more), unused once
+    // the nodes are packed: items + top indices take about 21 n bytes, the new
+    // nodes + slots about 33 n bytes
+    SahItem* d_items = reinterpret_cast<SahItem*>(w.boxes);
+    int* d_top = reinterpret_cast<int*>(d_items + cap);
+    unsigned* d_cnt = w.bounds + 12;  // two free words of the bounds block
+    HZ_HIP(hipMemsetAsync(d_cnt, 0, 2 * sizeof(unsigned), st), err);
+    hipLaunchKernelGGL(k_sah_cut, dim3(grid(n - 1)), dim3(kBlock), 0, st, w.range, w.child, n, T,
+                       out.nodes, d_items, d_top, d_cnt, cap);
+    HZ_HIP(hipGetLastError(), err);
+    unsigned cnt[2] = {0u, 0u};
+    HZ_HIP(hipMemcpyAsync(cnt, d_cnt, sizeof(cnt), hipMemcpyDeviceToHost, st), err);
+    HZ_HIP(hipStreamSynchronize(st), err);
+    const int ntop = (int)cnt[0], nitem = (int)cnt[1];
+    if (nitem > cap || ntop + 1 != nitem || ntop < 1) {
+        err = "SAH top rebuild: inconsistent cut (" + std::to_string(ntop) + " top nodes, " +
+              std::to_string(nitem) + " items)";
+        return -2;
+    }
+    std::vector<SahItem> items(nitem);
+    std::vector<int> top(ntop);
+    HZ_HIP(hipMemcpyAsync(items.data(), d_items, nitem * sizeof(SahItem), hipMemcpyDeviceToHost,
+                          st), err);
+    HZ_HIP(hipMemcpyAsync(top.data(), d_top, ntop * sizeof(int), hipMemcpyDeviceToHost, st), err);
+    HZ_HIP(hipStreamSynchronize(st), err);
+    std::vector<int> idx(nitem);
+    for (int k = 0; k < nitem; k++) idx[k] = k;
+    SahBuild b(items);
+    b.kids.reserve(ntop);
+    b.build(idx, 0, nitem);
+    if ((int)b.kids.size() != ntop) {
+        err = "SAH top rebuild: node count mismatch";
+        return -2;
+    }
+    // slots: new node 0 (the root) -> slot 0, the others -> the other old top slots
+    std::sort(top.begin(), top.end());
+    if (top[0] != 0) {
+        err = "SAH top rebuild: the root is not a top node";
+        return -2;
+    }
+    std::vector<BvhNode> nn(ntop);
+    for (int k = 0; k < ntop; k++) {
+        BvhNode& nd = nn[k];
+        std::memset(&nd, 0, sizeof(nd));
+        const int kid[2] = {b.kids[k].x, b.kids[k].y};
+        float bx[2][6];
+        int ref[2];
+        for (int s = 0; s < 2; s++) {
+            const int r = kid[s];
+            if (r >= 0) {
+                std::memcpy(bx[s], &b.box[(size_t)r * 6], sizeof(bx[s]));
+                ref[s] = top[r] | (b.wide[r] ? kWideBit : 0);
+            } else {
+                const SahItem& x = items[~r];
+                std::memcpy(bx[s], x.lo, 3 * sizeof(float));
+                std::memcpy(bx[s] + 3, x.hi, 3 * sizeof(float));
+                ref[s] = x.ref;
+                const int code = ~x.ref;
+                if (x.ref < 0 && (code >> kLeafFirstBits) == 0)
+                    std::memcpy(&nd.tri[9 * s], x.tri, 9 * sizeof(float));
+            }
+        }
+        nd.lx0 = bx[0][0];
+        nd.lx1 = bx[1][0];
+        nd.hx0 = bx[0][3];
+        nd.hx1 = bx[1][3];
+        nd.ly0 = bx[0][1];
+        nd.ly1 = bx[1][1];
+        nd.hy0 = bx[0][4];
+        nd.hy1 = bx[1][4];
+        nd.lz0 = bx[0][2];
+        nd.lz1 = bx[1][2];
+        nd.hz0 = bx[0][5];
+        nd.hz1 = bx[1][5];
+        nd.r0 = ref[0];
+        nd.r1 = ref[1];
+    }
+    // upload through the same scratch (items are no longer needed there)
+    BvhNode* d_nn = reinterpret_cast<BvhNode*>(w.boxes);
+    int* d_slot = reinterpret_cast<int*>(d_nn + ntop);
+    HZ_HIP(hipMemcpyAsync(d_nn, nn.data(), ntop * sizeof(BvhNode), hipMemcpyHostToDevice, st),
+           err);
+    HZ_HIP(hipMemcpyAsync(d_slot, top.data(), ntop * sizeof(int), hipMemcpyHostToDevice, st), err);
+    hipLaunchKernelGGL(k_sah_scatter, dim3(grid(ntop)), dim3(kBlock), 0, st, d_nn, d_slot, ntop,
+                       out.nodes);
+    HZ_HIP(hipGetLastError(), err);
+    HZ_HIP(hipStreamSynchronize(st), err);
+    return 0;
+}
+
 int lbvh_reserve(LbvhWork& w, int n, std::string& err) {
     if (n <= w.capacity) return 0;
     if (w.borrowed) {
@@ -761,7 +1041,7 @@ void lbvh_free(LbvhWork& w) {
 
 int lbvh_build(LbvhWork& w, const float* vert, const int* tri, int n, int leaf_size,
                int morton_mode, int width, hipStream_t st, DeviceBvh& out, bool want_depth,
-               std::string& err) {
+               std::string& err, bool sah_top) {
     out.num_tri = n;
     out.num_nodes = 0;
     out.depth = 0;
@@ -844,6 +1124,10 @@ int lbvh_build(LbvhWork& w, const float* vert, const int* tri, int n, int leaf_s
     }
     hipLaunchKernelGGL(k_tris, dim3(grid(n)), dim3(kBlock), 0, st, sorted, n, vert, tri,
                        out.tris);
+    if (sah_top && out.width == 2) {
+        const int rc = lbvh_sah_top(w, n, leaf_size, st, out, err);
+        if (rc) return rc;
+    }
     if (want_depth) {
         hipLaunchKernelGGL(k_depth, dim3(grid(n)), dim3(kBlock), 0, st, w.parent, n,
                            w.bounds + 6);

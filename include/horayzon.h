@@ -80,7 +80,11 @@ typedef struct hz_options {
                              uploads its scene; a call with other sizes frees them first.
                              0 = free everything at the end of the call.  hz_cache_release
                              frees what is kept.  The entry points without options use 1 */
-    int reserved[1];
+    int bvh_sah;          /* 1: rebuild the top of the LBVH (the nodes over more than
+                             max(leaf_size, 8, triangles / 8192) triangles) with a binned
+                             surface-area heuristic over the subtrees below it (binary trees
+                             of 4096 or more triangles; the build then waits for the host
+                             once); 0 (default): the Morton tree.  Same results either way */
 } hz_options;
 
 typedef struct hz_stats {

@@ -8,7 +8,10 @@ four-wide collapse), read back with hz_domain_bvh and walked on the host:
 * inline triangles equal the leaf-order array, and the "wide" bit on a
   reference says exactly whether the node holds inline triangles;
 * the leaf order is a permutation of the mesh's triangles, with their
-  vertices (hz_scene_export) unchanged.
+  vertices (hz_scene_export) unchanged;
+* all of this also holds with the top of the binary tree rebuilt by the
+  SAH (option bvh_sah), which returns the same horizons, rays and SVF as
+  the Morton tree on C1 and C2.
 The query results do not depend on the tree (test_gpu_parity.py); this
 pins the tree itself.
 """
@@ -131,12 +134,13 @@ def c1_dom(c1):
         yield dom, scene
 
 
-@pytest.mark.parametrize("width,leaf,morton", [(2, 1, 0), (2, 4, 0), (2, 1, 1), (4, 1, 0),
-                                               (4, 2, 0), (4, 8, 1)])
-def test_tree_structure(c1_dom, width, leaf, morton):
-    print(f"case bvh c1 width {width} leaf {leaf} morton {morton}", flush=True)
+@pytest.mark.parametrize("width,leaf,morton,sah", [(2, 1, 0, 0), (2, 4, 0, 0), (2, 1, 1, 0),
+                                                   (4, 1, 0, 0), (4, 2, 0, 0), (4, 8, 1, 0),
+                                                   (2, 1, 0, 1), (2, 4, 1, 1)])
+def test_tree_structure(c1_dom, width, leaf, morton, sah):
+    print(f"case bvh c1 width {width} leaf {leaf} morton {morton} sah {sah}", flush=True)
     dom, scene = c1_dom
-    nodes, tris = dom.bvh(bvh_width=width, leaf_size=leaf, bvh_morton=morton)
+    nodes, tris = dom.bvh(bvh_width=width, leaf_size=leaf, bvh_morton=morton, bvh_sah=sah)
     ntri = scene["tri"].shape[0]
     assert tris.shape[0] == ntri and nodes.shape[0] >= 1
     seen, visits = (_check_binary if width == 2 else _check_four)(nodes, tris)
@@ -174,3 +178,36 @@ def test_single_triangle_tree():
             seen, visits = (_check_binary if width == 2 else _check_four)(nodes, tris)
             # the binary root of a one-triangle tree holds it in both slots
             assert seen.tolist() == [2 if width == 2 else 1] and visits.tolist() == [1]
+
+
+def test_sah_top_same_results(c1, c2):
+    # the SAH-rebuilt top changes which nodes a ray visits, never its answer
+    for name, dom in (("c1", c1), ("c2", c2)):
+        with ResidentDomain(*dom.args(), grid_type=1, ray_org_elev=0.2) as rd:
+            for mode in (0, 1):
+                st0 = rd.run(24, 40000.0, 1, 2, search_mode=mode, certify=0, collect_counters=1)
+                h0, s0 = rd.fetch()
+                st1 = rd.run(24, 40000.0, 1, 2, search_mode=mode, certify=0, collect_counters=1,
+                             bvh_sah=1)
+                h1, s1 = rd.fetch()
+                print(f"case sah {name} mode {mode}: nodes per query "
+                      f"{st0['node_visits'] / st0['num_rays']:.2f} (Morton) "
+                      f"{st1['node_visits'] / st1['num_rays']:.2f} (SAH top)", flush=True)
+                assert np.array_equal(h0, h1) and np.array_equal(s0, s1)
+                assert st0["num_rays"] == st1["num_rays"]
+        if name == "c2":
+            with ResidentDomain(*dom.args(), grid_type=1, ray_org_elev=0.2) as rd:
+                a = rd.run(24, 40000.0, 1, 2)["t_trace_ms"]
+                b = rd.run(24, 40000.0, 1, 2, bvh_sah=1)["t_trace_ms"]
+                a = min(a, rd.run(24, 40000.0, 1, 2)["t_trace_ms"])
+                b = min(b, rd.run(24, 40000.0, 1, 2, bvh_sah=1)["t_trace_ms"])
+                print(f"c2 walk trace: {a:.2f} ms (Morton), {b:.2f} ms (SAH top)", flush=True)
+
+
+def test_sah_option_checks(c1_dom):
+    from horayzon_extpar_amd._lib import HzError
+    dom, _ = c1_dom
+    with pytest.raises(HzError):
+        dom.run(24, 40000.0, 1, 2, bvh_sah=1, bvh_width=4)
+    with pytest.raises(HzError):
+        dom.run(24, 40000.0, 1, 2, bvh_sah=2)
