@@ -159,6 +159,9 @@ struct hz_domain {
     bool bvh_pre = false;
     float bvh_pre_ms = 0.0f;
     hipEvent_t pre_ev[2] = {nullptr, nullptr};
+    double* d_pcell = nullptr;  // FP64 ENU cell points of a reload (frames built on the device)
+    int* d_holes = nullptr;     // cells next to mesh holes of a reload
+    size_t holes_cap = 0;
 };
 
 // The progress report the reference prints to stdout (mo_lradtopo_horayzon.cpp
@@ -243,7 +246,8 @@ void hz_domain_destroy(hz_domain* d) {
                     d->d_tris,   d->d_horizon, d->d_skyview, d->d_state,
                     d->d_spill,  d->d_elev_cs, d->d_desc,    d->d_pred,
                     d->d_adj_off, d->d_adj,    d->d_slope,   d->d_hitem,
-                    d->d_state2, d->d_state3, d->d_ranges, d->d_spill_x};
+                    d->d_state2, d->d_state3, d->d_ranges, d->d_spill_x,
+                    d->d_pcell,  d->d_holes};
     d->stager.release();
     for (void* b : bufs)
         if (b) (void)hipFree(b);
@@ -1051,8 +1055,10 @@ void cache_put(hz_domain* d) {
 }
 
 // Load a host scene of the same sizes into a kept domain: upload the FP32
-// scene, start the default tree build on the device, compute the cell frames
-// on the host meanwhile (prologue_frames), then upload them.  What was
+// scene and start the default tree build on the device, then upload the FP64
+// ENU cell points and build the cell frames from them on the device (the
+// host's frame arithmetic, the same bits), with the hole flags from the
+// host's list (hole_cells).  What was
 // derived from the previous scene (certificate index, slope adjacency,
 // predictor, cell flags, run state) is dropped.
 int domain_reload(hz_domain* d, HostScene& hs, const double* hsurf, const int* cov,
@@ -1096,12 +1102,35 @@ int domain_reload(hz_domain* d, HostScene& hs, const double* hsurf, const int* c
         if (rc) return fail(rc == -3 ? HZ_ENOMEM : HZ_EDEVICE, err);
         e = hipEventRecord(d->pre_ev[1], d->stream);
     }
-    const double tf0 = now_ms();
-    prologue_frames(cov, num_cell, num_vertex, ray_org_elev, hs);
-    const double tf1 = now_ms();
+    // the cell frames: built on the device from the FP64 ENU cell points (the
+    // host's frame arithmetic, same bits), then the hole flags from the host's
+    // list of cells next to mesh holes
+    if (e == hipSuccess && !d->d_pcell)
+        e = hipMalloc(&d->d_pcell, (size_t)num_cell * 3 * sizeof(double));
     if (e == hipSuccess)
-        e = d->stager.copy(d->d_cells, hs.frame.data(), (size_t)num_cell * sizeof(CellFrame), true,
+        e = d->stager.copy(d->d_pcell, hs.work.data(), (size_t)num_cell * 3 * sizeof(double), true,
                            d->stream);
+    if (e == hipSuccess)
+        launch_frames_from_enu(d->d_pcell, num_cell, hs.centre, hs.pole, ray_org_elev, d->d_cells,
+                               d->stream);
+    const double tf0 = now_ms();
+    hole_cells(cov, num_cell, num_vertex, hs.holes);
+    const double tf1 = now_ms();
+    const int nh = (int)hs.holes.size();
+    if (e == hipSuccess && (size_t)nh > d->holes_cap) {
+        if (d->d_holes) e = hipFree(d->d_holes);
+        d->d_holes = nullptr;
+        d->holes_cap = 0;
+        if (e == hipSuccess) e = hipMalloc(&d->d_holes, (size_t)nh * sizeof(int));
+        if (e == hipSuccess) d->holes_cap = (size_t)nh;
+    }
+    if (e == hipSuccess && nh > 0)
+        e = hipMemcpyAsync(d->d_holes, hs.holes.data(), (size_t)nh * sizeof(int),
+                           hipMemcpyHostToDevice, d->stream);
+    if (e == hipSuccess) {
+        launch_flag_cells(d->d_holes, nh, d->d_cells, d->stream);
+        e = hipGetLastError();
+    }
     if (e == hipSuccess) e = hipStreamSynchronize(d->stream);
     if (e == hipSuccess) e = hipEventElapsedTime(&d->bvh_pre_ms, d->pre_ev[0], d->pre_ev[1]);
     if (e != hipSuccess)

@@ -205,6 +205,7 @@ struct HostScene {
     HostVec<double> work;   // FP64 mesh vertices (lon/lat/h, then ECEF, then ENU)
     HostVec<int> vcount;    // per ICON vertex: triangles, added mesh vertices
     HostVec<long long> voff;  // their prefix sums
+    HostVec<int> holes;       // cells next to mesh holes (hole_cells)
     double centre[3] = {0.0, 0.0, 0.0};  // earth centre and north pole in ENU
     double pole[3] = {0.0, 0.0, 0.0};
 };
@@ -222,6 +223,18 @@ int prologue_scene(const double* clon, const double* clat, const double* hsurf,
                    std::string& err);
 void prologue_frames(const int* cells_of_vertex, int num_cell, int num_vertex,
                      double ray_org_elev, HostScene& out);
+// The cells next to mesh holes (corners of an ICON vertex with fewer than six
+// cells), the cells prologue_frames flags with bit 0; a cell may repeat.
+void hole_cells(const int* cells_of_vertex, int num_cell, int num_vertex, HostVec<int>& out);
+// The frames of prologue_frames computed on the device from the FP64 ENU cell
+// points P (3 per cell, as prologue_scene leaves them), with the same FP64
+// operations in the same order (IEEE add, multiply, divide and square root:
+// the same bits), flags clear; launch_flag_cells then sets flag bit 0 on
+// the listed cells.  Enqueued on st.
+void launch_frames_from_enu(const double* P, int ncell, const double centre[3],
+                            const double pole[3], double ray_org_elev, CellFrame* cells,
+                            hipStream_t st);
+void launch_flag_cells(const int* list, int n, CellFrame* cells, hipStream_t st);
 
 WalkParams make_walk_params(int num_hori, int refine_factor, int svf_type, double dist_search,
                             int ncell);

@@ -405,6 +405,20 @@ void prologue_frames(const int* cov, int ncell, int nvert, double ray_org_elev,
     }
 }
 
+void hole_cells(const int* cov, int ncell, int nvert, HostVec<int>& out) {
+    out.clear();
+    for (int v = 0; v < nvert; v++) {
+        int n = 0;
+        for (int j = 0; j < 6; j++)
+            if (cov[(size_t)nvert * j + v] != -2) n++;
+        if (n == 6) continue;
+        for (int j = 0; j < 6; j++) {
+            const int c = cov[(size_t)nvert * j + v];
+            if (c >= 0 && c < ncell) out.push_back(c);
+        }
+    }
+}
+
 void cell_adjacency(const int* tri, long long ntri, int ncell, std::vector<int>& off,
                     std::vector<int>& adj) {
     off.assign((size_t)ncell + 1, 0);

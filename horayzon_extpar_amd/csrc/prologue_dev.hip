@@ -424,6 +424,32 @@ free_tmp:
     return rc;
 }
 
+namespace {
+// Flag bit 0 on the listed cells (the host's list of cells next to mesh holes).
+__global__ void k_flag_cells(const int* __restrict__ list, int n, CellFrame* __restrict__ cells) {
+    const int i = blockIdx.x * kBlockP + threadIdx.x;
+    if (i < n) cells[list[i]].pad |= 1;
+}
+}  // namespace
+
+void launch_flag_cells(const int* list, int n, CellFrame* cells, hipStream_t st) {
+    if (n > 0)
+        hipLaunchKernelGGL(k_flag_cells, dim3(gridp(n)), dim3(kBlockP), 0, st, list, n, cells);
+}
+
+void launch_frames_from_enu(const double* P, int ncell, const double centre[3],
+                            const double pole[3], double ray_org_elev, CellFrame* cells,
+                            hipStream_t st) {
+    EnuFrame E;
+    std::memset(&E, 0, sizeof(E));
+    for (int a = 0; a < 3; a++) {
+        E.centre[a] = centre[a];
+        E.pole[a] = pole[a];
+    }
+    hipLaunchKernelGGL(k_frames, dim3(gridp(ncell)), dim3(kBlockP), 0, st, P, ncell, E,
+                       ray_org_elev, cells);
+}
+
 int sincos_cr_eval(const double* x, long long n, double* s, double* c, int device,
                    std::string& err) {
     if (n <= 0) return 0;

@@ -46,3 +46,18 @@ def test_cache_option_checked():
     a = synthetic.config1(seed=11)
     with pytest.raises(_lib.HzError):
         _run(a, 2)
+
+
+def test_kept_buffers_c2(c2):
+    # the reload path (tree built while the host works, cell frames built on
+    # the device from the host's FP64 ENU points) on the full C2 grid: the
+    # same bits as a call that keeps nothing
+    print("case cache c2", flush=True)
+    _lib.lib().hz_cache_release()
+    h0, s0, _ = _run(c2, 0)
+    _run(c2, 1)
+    h1, s1, st = _run(c2, 1)  # reuses the kept domain
+    _lib.lib().hz_cache_release()
+    assert np.array_equal(h0, h1) and np.array_equal(s0, s1)
+    print(f"c2 kept: prologue {st['t_prologue_ms']:.2f} ms, upload {st['t_upload_ms']:.2f} ms, "
+          f"bvh {st['t_bvh_ms']:.2f} ms", flush=True)
