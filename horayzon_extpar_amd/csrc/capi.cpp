@@ -706,16 +706,30 @@ int hz_domain_run(hz_domain* d, int num_hori, double dist_search, int refine_fac
     const bool cert_run = certify && hitem;
     const float tfar = (float)dist_search;
     if (cert_run) {
+        // the side streams and events of the certified searches
+        if (!d->bev[0]) {
+            hipError_t e_ = hipSuccess;
+            for (int i = 0; i < 2 && e_ == hipSuccess; i++) e_ = hipEventCreate(&d->bev[i]);
+            for (int i = 0; i < 4 && e_ == hipSuccess; i++) e_ = hipEventCreate(&d->cev[i]);
+            if (e_ == hipSuccess) e_ = hipStreamCreateWithFlags(&d->xstream, hipStreamNonBlocking);
+            if (e_ == hipSuccess) e_ = hipStreamCreateWithFlags(&d->ystream, hipStreamNonBlocking);
+            if (e_ != hipSuccess)
+                return fail(HZ_EDEVICE, std::string("certified search streams: ") +
+                                            hipGetErrorString(e_));
+        }
         // mesh-boundary index, then the cell classes (flag bit 1: a boundary
-        // point in reach) that the trace order sorts by
+        // point in reach) that the trace order sorts by; the classes run on a
+        // side stream beside the tree build, and the ordering waits for them
         rc = cert_index_build(d->cert, d->d_cells, d->d_vert, d->ncell, d->d_tri, d->ntri,
                               d->bounds_xy, d->hmin, d->hmax, tfar, st, err);
         if (rc) return fail(rc == -3 ? HZ_ENOMEM : HZ_EDEVICE, err);
         if (!d->d_ranges) RCHK(hipMalloc(&d->d_ranges, 10 * sizeof(int)));
-        RCHK(hipEventRecord(d->ev[5], st));
-        launch_cert_classes(d->cert, d->d_cells, d->ncell, false, d->d_ranges, st);
+        RCHK(hipEventRecord(d->cev[2], st));
+        RCHK(hipStreamWaitEvent(d->xstream, d->cev[2], 0));
+        RCHK(hipEventRecord(d->ev[5], d->xstream));
+        launch_cert_classes(d->cert, d->d_cells, d->ncell, false, d->d_ranges, d->xstream);
         RCHK(hipGetLastError());
-        RCHK(hipEventRecord(d->ev[6], st));
+        RCHK(hipEventRecord(d->ev[6], d->xstream));
         d->near_flags = true;
     } else if (d->near_flags) {  // the plain order again (bit 0 first, then Morton)
         if (!d->d_ranges) RCHK(hipMalloc(&d->d_ranges, 10 * sizeof(int)));
@@ -735,6 +749,7 @@ int hz_domain_run(hz_domain* d, int num_hori, double dist_search, int refine_fac
         if (rc) return fail(rc == -3 ? HZ_ENOMEM : (rc == -1 ? HZ_EINVAL : HZ_EDEVICE), err);
     }
     RCHK(hipEventRecord(d->ev[1], st));
+    if (cert_run) RCHK(hipStreamWaitEvent(st, d->ev[6], 0));  // the cell classes
     const int* order = nullptr;
     if (o.cell_order) {
         rc = order_cells(d->work, d->d_cells, d->ncell, d->d_order, st, err);
@@ -776,16 +791,6 @@ int hz_domain_run(hz_domain* d, int num_hori, double dist_search, int refine_fac
         // overlap the lattice items: C2 learned 21.5 ms against 31.6 ms for
         // the three in sequence.
         multi_state = true;
-        if (!d->bev[0]) {
-            hipError_t e_ = hipSuccess;
-            for (int i = 0; i < 2 && e_ == hipSuccess; i++) e_ = hipEventCreate(&d->bev[i]);
-            for (int i = 0; i < 4 && e_ == hipSuccess; i++) e_ = hipEventCreate(&d->cev[i]);
-            if (e_ == hipSuccess) e_ = hipStreamCreateWithFlags(&d->xstream, hipStreamNonBlocking);
-            if (e_ == hipSuccess) e_ = hipStreamCreateWithFlags(&d->ystream, hipStreamNonBlocking);
-            if (e_ != hipSuccess)
-                return fail(HZ_EDEVICE, std::string("certified search streams: ") +
-                                            hipGetErrorString(e_));
-        }
         if (!d->d_state2) RCHK(hipMalloc(&d->d_state2, sizeof(TraceState)));
         if (!d->d_state3) RCHK(hipMalloc(&d->d_state3, sizeof(TraceState)));
         if (d->spill_x_doubles < d->spill_doubles) {  // the far items' spill area
