@@ -4,7 +4,9 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
+#include <functional>
 #include <string>
 #include <memory>
 #include <utility>
@@ -254,6 +256,20 @@ void enu_frame_from_sum(const double* P, int ncell, EnuFrame& E);
 // from the cell lon / lat / elevation (host sin / cos, as the host prologue)
 void enu_frame_host(const double* clon, const double* clat, const double* hsurf, int ncell,
                     EnuFrame& E);
+
+// Host worker pool (pool.cpp).  pool_submit queues f as part of group grp;
+// pool_wait returns once every item of grp has run, running queued items
+// itself meanwhile.  Items must not wait on other items.  pool_for runs
+// f(b, e) over [0, n) in up to pool_threads() ranges of at least
+// min_per_task indices, the caller taking the first.
+struct TaskGroup {
+    std::atomic<int> left{0};
+};
+int pool_threads();
+void pool_submit(TaskGroup& grp, std::function<void()> f);
+void pool_wait(TaskGroup& grp);
+void pool_for(long long n, const std::function<void(long long, long long)>& f,
+              long long min_per_task = 4096);
 
 // Large host <-> device copies through pinned slots (stage.cpp): copy()
 // moves `bytes` between pageable host memory and device memory on stream

@@ -8,15 +8,15 @@
 //   per-cell frame / origin  :693-722
 // with the same arithmetic (operation order included), so that the FP32
 // vertices and ray origins handed to the device are the values the reference
-// hands to Embree (:406-410, :715-720).  Work is split over host threads; the
-// only order-dependent reduction (the ENU origin sum, :627-631) stays
-// sequential in cell order.
+// hands to Embree (:406-410, :715-720).  Work is split over the library's
+// host worker pool; the only order-dependent reduction (the ENU origin sum,
+// :627-631) stays sequential in cell order and runs on the calling thread
+// while the pool builds the triangles.
 
 #include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <mutex>
-#include <thread>
 
 #include "hz_internal.h"
 
@@ -34,23 +34,14 @@ inline double deg2rad(double a) { return (a / 180.0) * kPi; }
 // bits of separate sin and cos calls.
 inline void sin_cos(double x, double* s, double* c) { ::sincos(x, s, c); }
 
+// f(b, e) over [0, n) on the library's worker pool (small loops run here)
 template <class F>
 void parallel_for(long long n, F&& f) {
-    unsigned hw = std::thread::hardware_concurrency();
-    int nt = (int)std::min<unsigned>(hw ? hw : 1, 16);
-    if (n < 20000) nt = 1;
-    if (nt <= 1) {
+    if (n < 20000) {
         f(0LL, n);
         return;
     }
-    std::vector<std::thread> th;
-    long long chunk = (n + nt - 1) / nt;
-    for (int t = 0; t < nt; t++) {
-        long long b = t * chunk, e = std::min(n, b + chunk);
-        if (b >= e) break;
-        th.emplace_back([&f, b, e]() { f(b, e); });
-    }
-    for (auto& x : th) x.join();
+    pool_for(n, f);
 }
 
 // Valid adjacent cells of ICON vertex v, ordered clockwise from the
@@ -72,7 +63,7 @@ inline int sorted_cells(const double* clon, const double* clat, const double* vl
     int n = 0;
     double hs = 0.0;
     bool exact = true;
-    constexpr double kSep = 1e-9;
+    constexpr double kSep2 = 1e-18;  // (1e-9 rad)^2, compared with squared quantities
     for (int j = 0; j < 6; j++) {
         int c = cov[(size_t)nvert * j + v];
         if (c == -2) continue;
@@ -81,7 +72,7 @@ inline int sorted_cells(const double* clon, const double* clat, const double* vl
         dx[n] = x;
         r2[n] = x * x + y * y;
         half[n] = (y > 0.0 || (y == 0.0 && x >= 0.0)) ? 0 : 1;
-        if (!(std::fabs(y) > kSep * std::sqrt(r2[n]))) exact = false;
+        if (!(y * y > kSep2 * r2[n])) exact = false;
         c6[n] = c;
         hs += hsurf[c];
         n++;
@@ -89,7 +80,8 @@ inline int sorted_cells(const double* clon, const double* clat, const double* vl
     for (int i = 0; exact && i < n; i++)
         for (int k = i + 1; k < n; k++)
             if (half[i] == half[k] &&
-                !(std::fabs(dx[i] * dy[k] - dy[i] * dx[k]) > kSep * std::sqrt(r2[i] * r2[k]))) {
+                !((dx[i] * dy[k] - dy[i] * dx[k]) * (dx[i] * dy[k] - dy[i] * dx[k]) >
+                  kSep2 * (r2[i] * r2[k]))) {
                 exact = false;
                 break;
             }
@@ -257,19 +249,28 @@ int prologue_scene(const double* clon, const double* clat, const double* hsurf,
         err = "mesh has more triangles than the library supports (2^28)";
         return -1;
     }
-    // mesh vertices as lon, lat, elevation (reference :234-238, :300-301)
+    // mesh vertices: the cell points (reference :234-238) straight to
+    // spherical ECEF (:158-169); the added vertices (:300-301) as lon, lat,
+    // elevation first
     HostVec<double>& P = out.work;  // every entry is written below
     P.resize((size_t)nmv * 3);
     out.tri.resize((size_t)ntri * 3);
+    const double R = kRadEarth;
+    auto ecef = [R](double lon, double lat, double h, double* p) {
+        double sl, cl, sp, cp;
+        sin_cos(lon, &sl, &cl);
+        sin_cos(lat, &sp, &cp);
+        p[0] = (R + h) * cp * cl;
+        p[1] = (R + h) * cp * sl;
+        p[2] = (R + h) * sp;
+    };
     parallel_for(ncell, [&](long long b, long long e) {
-        for (long long i = b; i < e; i++) {
-            P[3 * i] = clon[i];
-            P[3 * i + 1] = clat[i];
-            P[3 * i + 2] = hsurf[i];
-        }
+        for (long long i = b; i < e; i++) ecef(clon[i], clat[i], hsurf[i], &P[3 * i]);
     });
-    // --- pass 2: fill triangles (reference :243-312) -------------------------
-    parallel_for(nvert, [&](long long b, long long e) {
+    // --- pass 2: fill triangles (reference :243-312), on the pool, while
+    // this thread sums the cell points in cell order for the ENU origin
+    // (reference :624-634; pass 2 writes only the added vertices' entries)
+    auto pass2 = [&](long long b, long long e) {
         int cell[6];
         double hs;
         for (long long v = b; v < e; v++) {
@@ -294,27 +295,26 @@ int prologue_scene(const double* clon, const double* clat, const double* hsurf,
                 }
             }
         }
-    });
-    // --- spherical ECEF (reference :158-169) --------------------------------
-    const double R = kRadEarth;
-    parallel_for(nmv, [&](long long b, long long e) {
-        for (long long i = b; i < e; i++) {
-            double sl, cl, sp, cp;
-            sin_cos(P[3 * i], &sl, &cl);
-            sin_cos(P[3 * i + 1], &sp, &cp);
-            double h = P[3 * i + 2];
-            P[3 * i] = (R + h) * cp * cl;
-            P[3 * i + 1] = (R + h) * cp * sl;
-            P[3 * i + 2] = (R + h) * sp;
+    };
+    TaskGroup grp;
+    const int nt2 = nvert < 20000 ? 0 : pool_threads();
+    if (nt2 == 0) {
+        pass2(0, nvert);
+    } else {
+        const long long chunk = ((long long)nvert + nt2 - 1) / nt2;
+        for (int t = 0; t < nt2; t++) {
+            const long long b = t * chunk, e = std::min<long long>(nvert, b + chunk);
+            if (b < e) pool_submit(grp, [&pass2, b, e]() { pass2(b, e); });
         }
-    });
-    // --- ENU origin and transform (reference :178-201, :624-644) -----------
+    }
     EnuFrame E;
     enu_frame_from_sum(P.data(), ncell, E);
+    pool_wait(grp);
     out.lon0 = E.lon0;
     out.lat0 = E.lat0;
     const double so = E.so, co = E.co, sa = E.sa, ca = E.ca;
     const double x0 = E.x0, y0 = E.y0, z0 = E.z0;
+    // --- ENU (reference :178-201, :643-644) ----------------------------------
     auto enu = [&](double x, double y, double z, double* r) {
         double dx = x - x0, dy = y - y0, dz = z - z0;
         r[0] = -so * dx + co * dy;
@@ -329,11 +329,13 @@ int prologue_scene(const double* clon, const double* clat, const double* hsurf,
     parallel_for(nmv, [&](long long b, long long e) {
         float lb[4] = {3e38f, 3e38f, -3e38f, -3e38f};
         for (long long i = b; i < e; i++) {
+            double* p = &P[3 * i];
+            if (i >= ncell) ecef(p[0], p[1], p[2], p);  // an added vertex: ECEF first
             double r[3];
-            enu(P[3 * i], P[3 * i + 1], P[3 * i + 2], r);
-            P[3 * i] = r[0];
-            P[3 * i + 1] = r[1];
-            P[3 * i + 2] = r[2];
+            enu(p[0], p[1], p[2], r);
+            p[0] = r[0];
+            p[1] = r[1];
+            p[2] = r[2];
             // FP32 scene vertex (reference :406-410)
             const float x = (float)r[0], y = (float)r[1];
             out.vert32[3 * i] = x;

@@ -1,17 +1,16 @@
 // Large host <-> device copies between pageable caller memory and the device
 // through three 16 MB slots of pinned memory: asynchronous copies move whole
-// slots on the stream while up to 8 host threads copy between the caller's
-// memory and the other slots.  Pinned copies run near the PCIe rate and the
-// host side is parallel (it also takes the first-touch page faults of a
-// freshly allocated destination in parallel), where a plain copy from / to
-// pageable memory goes through one runtime staging buffer.
+// slots on the stream while items of the library's worker pool (up to 8 per
+// slot) copy between the caller's memory and the other slots.  Pinned copies
+// run near the PCIe rate and the host side is parallel (it also takes the
+// first-touch page faults of a freshly allocated destination in parallel),
+// where a plain copy from / to pageable memory goes through one runtime
+// staging buffer.
 
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
-#include <atomic>
 #include <cstring>
-#include <thread>
 #include <vector>
 
 #include "hz_internal.h"
@@ -65,45 +64,32 @@ hipError_t Stager::copy(void* dst, const void* src, size_t bytes, bool to_device
                                  : hipMemcpyAsync(sl(k), s + (size_t)k * kChunk, len(k), kind, st);
         return r != hipSuccess ? r : hipEventRecord(ev[k % kSlots], st);
     };
-    unsigned hw = std::thread::hardware_concurrency();
-    const int nt = (int)std::max(1u, std::min(hw ? hw : 1u, 8u));
-    std::atomic<int> go{-1};  // host side of chunks <= go may start
-    std::atomic<bool> abort{false};
-    std::vector<std::atomic<int>> done(nch);
-    for (auto& x : done) x.store(0);
-    std::vector<std::thread> th;
-    auto work = [&](int t) {
-        for (int k = 0; k < nch; k++) {
-            while (go.load(std::memory_order_acquire) < k) {
-                if (abort.load()) return;
-                std::this_thread::yield();
-            }
-            const size_t n = len(k), b = n / nt * t, en = t == nt - 1 ? n : n / nt * (t + 1);
-            if (to_device)
-                std::memcpy(sl(k) + b, s + (size_t)k * kChunk + b, en - b);
-            else
-                std::memcpy(d + (size_t)k * kChunk + b, sl(k) + b, en - b);
-            done[k].fetch_add(1, std::memory_order_release);
+    // host side of chunk k: nt slices, each a pool item that never waits
+    const int nt = std::max(1, std::min(pool_threads(), 8));
+    std::unique_ptr<TaskGroup[]> grp(new TaskGroup[nch]);
+    int submitted = 0;
+    auto host = [&](int k) {
+        const size_t n = len(k);
+        char* hd = to_device ? sl(k) : d + (size_t)k * kChunk;
+        const char* hs = to_device ? s + (size_t)k * kChunk : sl(k);
+        for (int t = 0; t < nt; t++) {
+            const size_t b = n / nt * t, en = t == nt - 1 ? n : n / nt * (t + 1);
+            if (b < en) pool_submit(grp[k], [hd, hs, b, en]() { std::memcpy(hd + b, hs + b, en - b); });
         }
-    };
-    try {
-        for (int t = 0; t < nt; t++) th.emplace_back(work, t);
-    } catch (...) {  // no threads: nothing was copied yet, copy directly
-        abort.store(true);
-        for (auto& x : th) x.join();
-        return plain();
-    }
-    auto wait_done = [&](int k) {
-        while (done[k].load(std::memory_order_acquire) < nt) std::this_thread::yield();
+        submitted = k + 1;
     };
     if (to_device) {
-        // host fills slot k (once its previous device copy is over), then the
-        // device copies it
+        // host fills slot k once the device copy of chunk k - kSlots out of it
+        // is over (up to kSlots chunks ahead); the device copies chunk k once
+        // its host part is done
+        int next = 0;
         for (int k = 0; k < nch && e == hipSuccess; k++) {
-            if (k >= kSlots) e = hipEventSynchronize(ev[k % kSlots]);
+            while (next < nch && next < k + kSlots && e == hipSuccess) {
+                if (next >= kSlots) e = hipEventSynchronize(ev[next % kSlots]);
+                if (e == hipSuccess) host(next++);
+            }
             if (e != hipSuccess) break;
-            go.store(k, std::memory_order_release);
-            wait_done(k);
+            pool_wait(grp[k]);
             e = issue(k);
         }
     } else {
@@ -113,15 +99,14 @@ hipError_t Stager::copy(void* dst, const void* src, size_t bytes, bool to_device
         for (int k = 0; k < nch && e == hipSuccess; k++) {
             e = hipEventSynchronize(ev[k % kSlots]);
             if (e != hipSuccess) break;
-            go.store(k, std::memory_order_release);
-            if (k + kSlots < nch) {
-                wait_done(k);
-                e = issue(k + kSlots);
-            }
+            host(k);
+            pool_wait(grp[k]);
+            if (k + kSlots < nch) e = issue(k + kSlots);
         }
     }
-    if (e != hipSuccess) abort.store(true);
-    for (auto& x : th) x.join();
+    // the items read and write the caller's memory and the slots: all of
+    // them have run before returning, also after an error
+    for (int k = 0; k < submitted; k++) pool_wait(grp[k]);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     return e;
 }
