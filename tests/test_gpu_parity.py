@@ -68,12 +68,14 @@ def test_c1_climb_mode(c1, num_hori, rf, svf_type):
                          [(1, 128, 0), (1, 64, 0), (0, 64, 64), (0, 256, 8), (0, 128, 1)])
 def test_kernel_variants_bit_exact(c1, kernel, block, refill):
     print(f"case c1 kernel={kernel} block={block} refill={refill}", flush=True)
+    st = {}
     h, s = horizon_svf_comp_py(*c1.args(), 24, 1, 40000.0, 0.2, 1, 2, trace_kernel=kernel,
-                               block_size=block, refill=refill or None)
-    ho, so, _ = oracle.horizon_svf(c1, grid_type=1)
+                               block_size=block, refill=refill or None, stats=st)
+    ho, so, sto = oracle.horizon_svf(c1, grid_type=1)
     _report(f"c1 kernel{kernel}", h, ho)
     assert np.array_equal(h, ho)
     assert np.max(np.abs(s - so)) <= SVF_TOL
+    assert st["num_rays"] == sto.num_rays
 
 
 @pytest.mark.parametrize("leaf,morton,width",
