@@ -154,10 +154,13 @@ def main():
                      "frac_measured_traffic_gather_scaled": (
                          traffic[1] / (tr * 1e-3) / 1e9 / PEAK_HBM_GBS
                          if traffic and traffic[1] else None),
-                     "limiter": "instruction issue under lane divergence (no unit "
-                                "saturated: VALU busy with a minority of lanes active, "
-                                "HBM below peak); the HBM roofline is the bound the "
-                                "kernel is priced against",
+                     "limiter": "co-limited by instruction issue under lane divergence "
+                                "(VALU 67-72% busy at 25% lane use) and the vector-memory "
+                                "data path (TD 85% busy, 43% waiting on L1 for node "
+                                "fetches that missed L2; HBM below peak); the HBM "
+                                "roofline is the bound the kernel is priced against",
+                     "limiter_source": "profiles/k_trace_c2_counters.md, "
+                                       "profiles/k_trace_c2_memory_path.md",
                      "kernel": "k_trace", "bytes_per_ray": b_ray,
                      "nodes_per_ray": n_node, "tris_per_ray": n_tri},
         "trace_counters": {"refill_frac": cnt["refill_frac"],
