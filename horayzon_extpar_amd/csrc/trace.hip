@@ -315,6 +315,136 @@ __device__ __forceinline__ int trav_step(const BvhNode* __restrict__ nodes,
     return 0;
 }
 
+// Binary tree with single-triangle leaves, lattice searches: the next node's
+// first half (child boxes and references, 64 bytes) is loaded straight into
+// LDS by wavefront DMA loads (16 bytes per lane per instruction, lane l's
+// bytes at base + 16 l), so a step can request it before its triangle tests
+// without holding it in registers.  Each wavefront has a 4 KB node buffer:
+// quarter k of lane l's first half at nb[k * 64 + l].
+__device__ __forceinline__ void dma_boxes(const BvhNode* __restrict__ nodes, int node,
+                                          float4* nb) {
+    const float4* np = reinterpret_cast<const float4*>(nodes + (node & kLeafFirstMask));
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+        __builtin_amdgcn_global_load_lds(static_cast<const void*>(np + k),
+                                         (__attribute__((address_space(3))) void*)(nb + k * 64),
+                                         16, 0, 0);
+}
+
+// trav_step for single-triangle leaves with the next fetch issued early.  In
+// an any-hit query the triangle tests only decide whether the query ends, so
+// the next node (nearer hit interior child, or the stack top) is known once
+// the child boxes are tested: its first half is requested into the node
+// buffer before the two inline-triangle test sites run, and the tests overlap
+// the fetch.  On entry the node buffer holds (or is loading) node `node`'s
+// first half; its second half (inline triangles, same 128-byte line) is read
+// into registers at the start of the step and is needed only after the box
+// tests and the stack.  The LDS stack holds the first DEP entries.  Same
+// return values, stack, hits and counts as trav_step; a fetch issued by a
+// step whose query then ends is never read.
+template <bool COUNT, int BLOCK, int DEP>
+__device__ __forceinline__ int trav_step_pf(const BvhNode* __restrict__ nodes, int* stk,
+                                            int* ovf, float4* nb, const Ray& r, int& node,
+                                            int& sp, unsigned& nvisit, unsigned& ntest, int& err,
+                                            int& hit_leaf) {
+    if (COUNT) nvisit++;
+    const int lane = lane_now();
+    // the node buffer is written by the DMA loads issued before this step: wait
+    // for them (the compiler does not order LDS reads after those loads across
+    // the loop, so the wait is explicit, and the clobber keeps the reads below it)
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    const float4 a = nb[lane], b = nb[64 + lane], c = nb[128 + lane], d = nb[192 + lane];
+    float4 e4, f4, g4, h4;  // left unset (never read) for nodes without inline triangles
+    if (node & kWideBit) {
+        const float4* np = reinterpret_cast<const float4*>(nodes + (node & kLeafFirstMask));
+        e4 = np[4];
+        f4 = np[5];
+        g4 = np[6];
+        h4 = np[7];
+    }
+    float x0l = __builtin_fmaf(a.x, r.ix, -r.oix), x0h = __builtin_fmaf(a.z, r.ix, -r.oix);
+    float x1l = __builtin_fmaf(a.y, r.ix, -r.oix), x1h = __builtin_fmaf(a.w, r.ix, -r.oix);
+    float y0l = __builtin_fmaf(b.x, r.iy, -r.oiy), y0h = __builtin_fmaf(b.z, r.iy, -r.oiy);
+    float y1l = __builtin_fmaf(b.y, r.iy, -r.oiy), y1h = __builtin_fmaf(b.w, r.iy, -r.oiy);
+    float z0l = __builtin_fmaf(c.x, r.iz, -r.oiz), z0h = __builtin_fmaf(c.z, r.iz, -r.oiz);
+    float z1l = __builtin_fmaf(c.y, r.iz, -r.oiz), z1h = __builtin_fmaf(c.w, r.iz, -r.oiz);
+    float tmin0 = fmaxf(fmaxf(fminf(x0l, x0h), fminf(y0l, y0h)), fmaxf(fminf(z0l, z0h), -1.0f));
+    float tmax0 = fminf(fminf(fmaxf(x0l, x0h), fmaxf(y0l, y0h)), fminf(fmaxf(z0l, z0h), r.tfar_box));
+    float tmin1 = fmaxf(fmaxf(fminf(x1l, x1h), fminf(y1l, y1h)), fmaxf(fminf(z1l, z1h), -1.0f));
+    float tmax1 = fminf(fminf(fmaxf(x1l, x1h), fmaxf(y1l, y1h)), fminf(fmaxf(z1l, z1h), r.tfar_box));
+    const bool h0 = tmin0 <= tmax0;
+    const bool h1 = tmin1 <= tmax1;
+    const int r0 = __float_as_int(d.x), r1 = __float_as_int(d.y);
+    const float t0 = d.z, t1 = d.w;                     // slot 0's first two coordinates
+    const bool l0 = h0 && r0 < 0, l1 = h1 && r1 < 0;    // hit leaf children
+    const bool i0 = h0 && r0 >= 0, i1 = h1 && r1 >= 0;  // hit interior children
+    // next node first
+    bool empty = false, over = false;
+    int next = 0;
+    if (i0 && i1) {
+        const bool first0 = tmin0 <= tmin1;
+        const int farc = first0 ? r1 : r0;
+        next = first0 ? r0 : r1;
+        if (sp >= kStackMax) {
+            over = true;
+        } else {
+            if (sp < DEP)
+                stk[sp * BLOCK + lane] = farc;
+            else
+                ovf[sp - DEP] = farc;
+            sp++;
+        }
+    } else if (i0) {
+        next = r0;
+    } else if (i1) {
+        next = r1;
+    } else if (sp == 0) {
+        empty = true;
+    } else {
+        sp--;
+        if (sp < DEP) {
+            next = stk[sp * BLOCK + lane];
+        } else {
+            next = ovf[sp - DEP];
+            // a wait the compiler sees (vmcnt 0): on this rare path the private
+            // load is the newest, and without it the merged wait before the
+            // request below would cover the inline-triangle loads on every path
+            __builtin_amdgcn_s_waitcnt(0xF70);
+        }
+    }
+    // the inline triangles have arrived before the request (a wait the
+    // compiler sees: the tests below then wait for nothing, where its own
+    // wait for them would also wait for the request); the request is made on
+    // every path (the root, a resident line, when the query ends here)
+    __builtin_amdgcn_s_waitcnt(0xF70);
+    // the buffer's reads of this step are done before the request overwrites it
+    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+    dma_boxes(nodes, (empty || over) ? 0 : next, nb);
+    // then this node's inline triangles (slot 0: d.zw e4 f4.xyz, slot 1:
+    // f4.w g4 h4), each at its own test site
+    if (l0) {
+        if (COUNT) ntest++;
+        if (tri_hit(r, t0, t1, e4.x, e4.y, e4.z, e4.w, f4.x, f4.y, f4.z)) {
+            hit_leaf = ~r0;  // a single-triangle leaf: its leaf-order index
+            return 1;
+        }
+    }
+    if (l1) {
+        if (COUNT) ntest++;
+        if (tri_hit(r, f4.w, g4.x, g4.y, g4.z, g4.w, h4.x, h4.y, h4.z, h4.w)) {
+            hit_leaf = ~r1;
+            return 1;
+        }
+    }
+    if (over) {
+        err |= 1;
+        return 1;
+    }
+    if (empty) return 2;
+    node = next;
+    return 0;
+}
+
 // One step of the any-hit traversal of the four-wide tree: fetch node `node`
 // (64 bytes, or 128 with its inline triangles), test the quantized boxes of
 // its children, test the triangles of hit leaf children at once, continue
@@ -692,7 +822,8 @@ static_assert(S_FIELDS == S_FIELDS_HOST, "spill layout");
 // and the SVF.
 // P.range (device {first, count}; NULL: all cells): the positions of the
 // trace order this launch works on (its cells, or the items of its cells).
-template <int MODE, bool COUNT, int BLOCK, int TREE, bool BATCH, bool ITEMS = false>
+template <int MODE, bool COUNT, int BLOCK, int TREE, bool BATCH, bool ITEMS = false,
+          bool PFW = false>
 __global__ __launch_bounds__(BLOCK, 4) void k_trace_pt(const BvhNode* __restrict__ nodes,
                                                     const float4* __restrict__ tris, int num_tri,
                                                     const CellFrame* __restrict__ cells,
@@ -705,10 +836,24 @@ __global__ __launch_bounds__(BLOCK, 4) void k_trace_pt(const BvhNode* __restrict
                                                     const BatchDomain* __restrict__ doms, int ndom,
                                                     int total, double* __restrict__ hitem) {
     static_assert(!ITEMS || (MODE != 0 && !BATCH), "items: lattice searches of one domain");
-    __shared__ int stk_all[kLdsStack * BLOCK];
+    // Lattice searches over the binary tree with single-triangle leaves step
+    // with the next node's boxes requested early (trav_step_pf), and so does
+    // the walk with PFW (a part of the cells beside another launch); a walk
+    // over all cells keeps the plain step, which is faster there (it fills
+    // the GPU's memory path alone, where the early request only adds work).
+    constexpr bool PF = TREE == 0 && (MODE != 0 || PFW);
+    // LDS: the stack's first kLdsDep entries per lane, and with PF a 64-byte
+    // node buffer per lane (a 16-entry stack then keeps the LDS at 128 bytes
+    // per lane; the deepest stack of a C2 query is 13 entries, deeper ones
+    // continue in ovf)
+    constexpr int kLdsDep = PF ? 16 : kLdsStack;
+    __shared__ int stk_all[kLdsDep * BLOCK];
+    __shared__ float4 nbuf_all[PF ? 4 * BLOCK : 1];
     // the wave's part of the stack: stk[k * BLOCK + lane] (the base is uniform)
-    int* stk = stk_all + (__builtin_amdgcn_readfirstlane(threadIdx.x) & ~63);
-    int ovf[kStackMax - kLdsStack];
+    const int wbase = __builtin_amdgcn_readfirstlane(threadIdx.x) & ~63;
+    int* stk = stk_all + wbase;
+    float4* nbuf = nbuf_all + (PF ? 4 * wbase : 0);
+    int ovf[kStackMax - kLdsDep];
     const int lane = threadIdx.x & 63;
     const unsigned long long lt_mask = (1ull << lane) - 1ull;
     const int L = P.num_hori * P.refine_factor;
@@ -734,6 +879,9 @@ __global__ __launch_bounds__(BLOCK, 4) void k_trace_pt(const BvhNode* __restrict
     // lane state kept in registers across the traversal (the rest of the
     // search state lives in the spill area)
     bool has_cell = false, live = false, last_hit = false;
+    // a query issued since the lane's last traversal step (binary tree with
+    // single-triangle leaves: its first node is not in the node buffer yet)
+    bool fresh = false;
     int hint = -1;  // leaf-order index of the triangle that occluded the cell's last hit
     Ray ray = make_ray(0.f, 0.f, 0.f, 1.f, 0.f, 0.f, P.tfar, P.tfar_box);
     int node = kWideBit, sp = 0, steps = 0;
@@ -1025,6 +1173,7 @@ __global__ __launch_bounds__(BLOCK, 4) void k_trace_pt(const BvhNode* __restrict
                     sp = 0;
                     steps = 0;
                     live = true;
+                    fresh = true;
                     if (MODE == 0) {
                         SV(S_DIR) = dir.x;
                         SV(S_DIR + 1) = dir.y;
@@ -1068,12 +1217,22 @@ __global__ __launch_bounds__(BLOCK, 4) void k_trace_pt(const BvhNode* __restrict
         // that still has queries to make does not wait for the wave's slowest
         int thresh = refill;
         if (exhausted) thresh = max(1, (__popcll(__ballot(has_cell)) * refill + 63) >> 6);
+        // PF: each step requests the next node's boxes before its triangle
+        // tests, so a lane whose query goes on from the last phase has them in
+        // its node buffer; a new query's root is requested here
+        if constexpr (PF) {
+            if (live && have && fresh) dma_boxes(lnodes, node, nbuf);
+            fresh = false;
+        }
         while (true) {
             if (COUNT) iters++;
             if (live) {
                 int s = 2;
                 if (have) {
-                    if constexpr (TREE == 2)
+                    if constexpr (PF)
+                        s = trav_step_pf<COUNT, BLOCK, kLdsDep>(lnodes, stk, ovf, nbuf, ray, node,
+                                                                sp, nvisit, ntest, err, hint);
+                    else if constexpr (TREE == 2)
                         s = trav_step4<COUNT, BLOCK>(reinterpret_cast<const Bvh4Node*>(lnodes),
                                                      ltris, stk, ovf, ray, node, sp, nvisit,
                                                      ntest, err, hint);
@@ -1345,7 +1504,8 @@ __global__ void k_items_finish(const double* __restrict__ hitem, const int* __re
     skyview[cell] = svf / (double)L;
 }
 
-template <int MODE, bool COUNT, int BLOCK, int TREE, bool BATCH, bool ITEMS = false>
+template <int MODE, bool COUNT, int BLOCK, int TREE, bool BATCH, bool ITEMS = false,
+          bool PFW = false>
 int pt_grid(int ncell, int reserve = 0) {
     static const int occ_grid = []() {
         int dev = 0, cus = 0, per = 0;
@@ -1354,7 +1514,8 @@ int pt_grid(int ncell, int reserve = 0) {
             return 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
                 &per,
-                reinterpret_cast<const void*>(&k_trace_pt<MODE, COUNT, BLOCK, TREE, BATCH, ITEMS>),
+                reinterpret_cast<const void*>(
+                    &k_trace_pt<MODE, COUNT, BLOCK, TREE, BATCH, ITEMS, PFW>),
                 BLOCK, 0) != hipSuccess)
             return 0;
         return std::max(1, per) * std::max(1, cus);
@@ -1380,6 +1541,17 @@ void launch_pt(const DeviceBvh& bvh, const CellFrame* cells, const int* order,
             if (finish)
                 hipLaunchKernelGGL(k_items_finish, dim3((p.ncell + 255) / 256), dim3(256), 0, st,
                                    hitem, order, p, horizon, skyview);
+            return;
+        }
+    }
+    if constexpr (MODE == 0 && TREE == 0) {
+        if (p.range) {  // the near cells' walk of the certified searches, beside the lattice
+            hipLaunchKernelGGL((k_trace_pt<MODE, COUNT, BLOCK, TREE, false, false, true>),
+                               dim3(pt_grid<MODE, COUNT, BLOCK, TREE, false, false, true>(
+                                   p.ncell)),
+                               dim3(BLOCK), 0, st, bvh.nodes, bvh.tris, bvh.num_tri, cells, order,
+                               p, refill, kChunk, horizon, skyview, ts, spill, nullptr, 0, 0,
+                               nullptr);
             return;
         }
     }
