@@ -36,7 +36,9 @@
 //
 // The traversal stack keeps its first kLdsStack entries in LDS
 // (entry-major, so lanes hit distinct banks) and deeper entries in a private
-// array.
+// array.  The lattice searches and the near-cell walk beside them step with
+// the next node's boxes requested early, into an LDS node buffer
+// (trav_step_pf, with 16 LDS stack entries).
 
 #include "hz_internal.h"
 
