@@ -58,6 +58,33 @@ def test_c1_certified_climb_equals_oracle_and_walk(c1, num_hori, rf, svf_type):
     assert np.abs(hc - hw).max() > 0.25
 
 
+# The lattice kernels (work items and whole cells) and the certified searches'
+# near-cell walk step with the next node's boxes requested early into a node
+# buffer per wavefront: every block size, refill thresholds, both schedules
+# (the certified search takes work items only).
+@pytest.mark.parametrize("block,whole,refill", [(64, 0, 0), (256, 0, 8), (64, 1, 64), (256, 1, 0)])
+def test_c1_lattice_block_sizes_equal_oracle(c1, block, whole, refill):
+    print(f"case c1 climb block={block} whole_cells={whole} refill={refill}", flush=True)
+    st = {}
+    h, s = horizon_svf_comp_py(*c1.args(), 24, 1, 40000.0, 0.2, 1, 2, search_mode=1, certify=0,
+                               block_size=block, whole_cells=whole, refill=refill or None,
+                               stats=st)
+    ho, so, sto = oracle.horizon_svf(c1, grid_type=1, mode=1)
+    assert np.array_equal(h, ho)
+    assert np.max(np.abs(s - so)) <= SVF_TOL
+    assert st["num_rays"] == sto.num_rays
+    if whole:
+        return
+    st = {}
+    h, s = horizon_svf_comp_py(*c1.args(), 24, 1, 40000.0, 0.2, 1, 2, search_mode=1,
+                               block_size=block, refill=refill or None, stats=st)
+    ho, so, sto = oracle.horizon_svf(c1, grid_type=1, mode=3)
+    assert np.array_equal(h, ho)
+    assert np.max(np.abs(s - so)) <= SVF_TOL
+    assert st["num_rays"] == sto.num_rays
+    assert int(st["cert_walk_cells"]) == sto.cells_walked
+
+
 def test_certified_learned_small_c2_equals_oracle():
     print("case small c2 certified learned", flush=True)
     dom = synthetic.config2(seed=21, width_km=120.0, height_km=90.0)
