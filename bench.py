@@ -400,7 +400,7 @@ def profiled_traffic(args):
     return (copy, gather,
             "profiles/k_trace_c2_pmc.md: rocprofv3 FETCH_SIZE (x 2.0 on a 1 GiB streaming "
             "copy; x 0.563 on a random 64-byte gather) + WRITE_SIZE x 1.0, "
-            "k_trace_pt<0, false, 128, 0, false, false>")
+            "k_trace_pt<0, false, 128, 0, false, false, false>")
 
 
 def accuracy_leg(dom_in, h, s, h_orc, s_orc, threads, nsample=48):
