@@ -317,7 +317,8 @@ __device__ __forceinline__ int trav_step(const BvhNode* __restrict__ nodes,
     return 0;
 }
 
-// Binary tree with single-triangle leaves, lattice searches: the next node's
+// Binary tree with single-triangle leaves, lattice searches and the walk of
+// the near cells beside them: the next node's
 // first half (child boxes and references, 64 bytes) is loaded straight into
 // LDS by wavefront DMA loads (16 bytes per lane per instruction, lane l's
 // bytes at base + 16 l), so a step can request it before its triangle tests
