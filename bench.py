@@ -118,6 +118,8 @@ def main():
     bytes_launch = rays * b_ray + ncell * (64 + 8 * (nhori + 1))
     achieved = bytes_launch / (tr * 1e-3) / 1e9
     traffic = profiled_traffic(args)
+    node_rate = cnt["node_visits"] / (tr * 1e-3)
+    gather = gather_ceiling(cnt["bvh_nodes"], node_rate) if args.kernel == 0 else None
     out = {
         "metric": "horizon+SVF cells/s @nhori=24, \u22640.1\u00b0 err; rays/s; learned vs plain "
                   "speedup",
@@ -154,11 +156,14 @@ def main():
                      "frac_measured_traffic_gather_scaled": (
                          traffic[1] / (tr * 1e-3) / 1e9 / PEAK_HBM_GBS
                          if traffic and traffic[1] else None),
-                     "limiter": "co-limited by instruction issue under lane divergence "
-                                "(VALU 67-72% busy at 25% lane use) and the vector-memory "
-                                "data path (TD 85% busy, 43% waiting on L1 for node "
-                                "fetches that missed L2; HBM below peak); the HBM "
-                                "roofline is the bound the kernel is priced against",
+                     "limiter": "the node fetches run at the GPU's rate for dependent "
+                                "random 64-byte gathers (gather_ceiling, measured in this "
+                                "run; it does not rise from 2 to 8 wavefronts per SIMD); "
+                                "symptoms: VALU 67-72% busy at 25% lane use, the vector "
+                                "data path 85% busy with 43% waiting on L1 for node fetches "
+                                "that missed L2, HBM below peak; the HBM roofline is the "
+                                "bound the kernel is priced against",
+                     "gather_ceiling": gather,
                      "limiter_source": "profiles/k_trace_c2_counters.md, "
                                        "profiles/k_trace_c2_memory_path.md",
                      "kernel": "k_trace", "bytes_per_ray": b_ray,
@@ -183,6 +188,24 @@ def main():
         if mode == 0:
             out["accuracy"] = accuracy_leg(dom_in, h, s, h_orc, s_orc, args.cpu_threads)
     print(json.dumps(out), flush=True)
+
+
+def gather_ceiling(num_nodes, node_rate):
+    """The GPU's rate for what bounds the walk's node fetches: dependent
+    random reads of 64 bytes from 128-byte records (hz_calibrate_gather), on
+    an array of the trace's tree size, at the trace kernel's occupancy, fully
+    random and with every other step in an 8 MB resident set (the walk's
+    upper tree levels stay in L2); the trace kernel's node fetches per
+    second against both."""
+    from horayzon_extpar_amd import _lib
+    full = _lib.calibrate_gather(num_nodes, num_nodes)
+    half = _lib.calibrate_gather(num_nodes, min(65536, num_nodes))
+    return {"unit": "node fetches/s", "achieved": node_rate, "random": full,
+            "half_in_8mb": half, "frac_random": node_rate / full,
+            "frac_half_in_8mb": node_rate / half,
+            "source": "hz_calibrate_gather: dependent random 64-byte reads of 128-byte "
+                      "records, four 16-byte loads per lane, 4 wavefronts per SIMD, 60% of "
+                      "lanes per step"}
 
 
 def end_to_end_leg(dom_in, opts, reps=3):

@@ -129,6 +129,8 @@ SIGNATURES = [
       ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_longlong)]),
     ("hz_sincos", ctypes.c_int,
      [c_double_p, ctypes.c_longlong, c_double_p, c_double_p, ctypes.c_int]),
+    ("hz_calibrate_gather", ctypes.c_int,
+     [ctypes.c_longlong, ctypes.c_longlong, ctypes.c_int, c_double_p]),
     ("hz_walk_noalloc", ctypes.c_int,
      [ctypes.c_void_p, ctypes.c_longlong, ctypes.c_void_p, ctypes.c_longlong, ctypes.c_void_p,
       ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double,
@@ -189,3 +191,14 @@ class HzError(RuntimeError):
 def check(rc: int):
     if rc != HZ_OK:
         raise HzError(rc, last_error())
+
+
+def calibrate_gather(num_nodes: int, hot_nodes: int, iters: int = 2000) -> float:
+    """Steps per second of dependent random 64-byte reads from `num_nodes`
+    128-byte records (`hot_nodes` of them take every other step), at the
+    trace kernel's occupancy: the GPU's random-gather rate that bounds the
+    walk's node fetches.  Runs on the current device."""
+    rate = ctypes.c_double(0.0)
+    check(lib().hz_calibrate_gather(int(num_nodes), int(hot_nodes), int(iters),
+                                    ctypes.byref(rate)))
+    return rate.value

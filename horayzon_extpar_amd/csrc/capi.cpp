@@ -472,6 +472,15 @@ int hz_domain_scene(const hz_domain* d, float* vert32, int* tri, double* frame_n
     return HZ_OK;
 }
 
+int hz_calibrate_gather(long long num_nodes, long long hot_nodes, int iters, double* rate) {
+    g_err.clear();
+    std::string err;
+    const int rc = gather_chase_rate(num_nodes, hot_nodes, iters, rate, err);
+    if (rc == -1) return fail(HZ_EINVAL, err);
+    if (rc) return fail(HZ_EDEVICE, err);
+    return HZ_OK;
+}
+
 int hz_sincos(const double* x, long long n, double* s, double* c, int on_device) {
     g_err.clear();
     if (n < 0 || (n > 0 && (!x || !s || !c))) return fail(HZ_EINVAL, "hz_sincos: bad arguments");

@@ -452,6 +452,12 @@ int launch_walk_team(const DeviceBvh& bvh, const CellFrame* cells, const int* or
 void launch_clear_state(TraceState* state, hipStream_t stream);
 size_t trace_spill_doubles(int nwork);
 
+// Rate (steps per second) of dependent random 64-byte reads from num_nodes
+// 128-byte records at the trace kernel's occupancy (calib.hip); hot_nodes of
+// them take every other step.  Allocates and frees its own device memory.
+int gather_chase_rate(long long num_nodes, long long hot_nodes, int iters, double* rate,
+                      std::string& err);
+
 // Error reporting and argument checks of the C-ABI (capi.cpp): api_fail sets
 // the message hz_last_error returns and passes code through.
 int api_fail(int code, const std::string& msg);

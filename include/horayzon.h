@@ -328,6 +328,18 @@ int hz_domain_bvh(hz_domain* dom, const hz_options* opt, void* nodes, long long*
 int hz_sincos(const double* x, long long n, double* s, double* c, int on_device);
 
 /*
+ * Measurement support, no counterpart in the reference: the rate, in steps
+ * per second, of dependent random reads of 64 bytes (a traversal step's node
+ * boxes) from num_nodes 128-byte records, at the trace kernel's occupancy
+ * and lane activity; hot_nodes of the records take every other step.  The
+ * trace kernel's node fetches per second against this rate place it on the
+ * GPU's random-gather roofline (bench.py).  Allocates num_nodes * 128 bytes
+ * of device memory for the call; num_nodes in [1024, 2^27], iters (steps
+ * per lane) in [1, 100000].
+ */
+int hz_calibrate_gather(long long num_nodes, long long hot_nodes, int iters, double* rate);
+
+/*
  * The reference walk without device allocation.  No counterpart in the
  * reference: for callers that cannot allocate device memory (code under
  * stream capture, tight serving loops, harnesses that forbid hipMalloc).

@@ -216,3 +216,13 @@ def test_fortran_variant_reports_into_buffer():
     assert "failed" in text and "num_hori" in text
     _, _, short = _fortran_call(d, num_hori=3, buf_len=20)
     assert len(short) == 20 and text.startswith(short)
+
+
+def test_calibrate_gather_checks_arguments():
+    # the checks come before any device work, so they hold without a GPU
+    from horayzon_extpar_amd._lib import calibrate_gather
+    for args in ((100, 10, 10), (1 << 16, 1 << 17, 10), (1 << 16, 0, 10), (1 << 16, 16, 0),
+                 (1 << 28, 16, 10)):
+        with pytest.raises(_lib.HzError) as e:
+            calibrate_gather(*args)
+        assert e.value.code == -1
