@@ -38,6 +38,10 @@ static_assert(sizeof(CellFrame) == 64, "CellFrame layout");
 //              holds an inline triangle (its second 64 bytes must be read)
 //   ref <  0 : leaf; ~ref = first | ((count - 1) << 28), triangles in leaf
 //              order; a leaf of count 1 is the inline triangle of its slot
+// A node whose children are both interior has no inline triangle; its
+// tri[1] then holds (as int bits) the leaf-order index of child 0's last
+// leaf, child 1's leaves following it, or -1 where the children's leaves are
+// not two such ranges (the top rebuilt with the SAH option).
 struct alignas(16) BvhNode {
     float lx0, lx1, hx0, hx1;
     float ly0, ly1, hy0, hy1;

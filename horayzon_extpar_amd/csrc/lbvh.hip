@@ -361,6 +361,10 @@ __global__ __launch_bounds__(kBlock) void k_pack(const int* __restrict__ sorted,
     nd.hz1 = grow_hi(h[1].z);
     nd.r0 = ref[0];
     nd.r1 = ref[1];
+    // both children interior (the node has no inline triangle): tri[1] holds
+    // the leaf-order index of child 0's last leaf, child 1's leaves follow it
+    // (the traversal descends first towards the triangle of the last hit)
+    if (ref[0] >= 0 && ref[1] >= 0) nd.tri[1] = __int_as_float(range[2 * child[2 * i] + 1]);
     nodes[i] = nd;
 }
 
@@ -934,6 +938,11 @@ static int lbvh_sah_top(LbvhWork& w, int n, int leaf_size, hipStream_t st, Devic
         nd.hz1 = bx[1][5];
         nd.r0 = ref[0];
         nd.r1 = ref[1];
+        // the children's leaves are not two leaf-order ranges here: no split
+        if (ref[0] >= 0 && ref[1] >= 0) {
+            const int none = -1;
+            std::memcpy(&nd.tri[1], &none, sizeof(none));
+        }
     }
     // upload through the same scratch (items are no longer needed there)
     BvhNode* d_nn = reinterpret_cast<BvhNode*>(w.boxes);

@@ -246,10 +246,24 @@ __device__ __forceinline__ bool first_child0(const Ray& r, float ztop0, float zt
 // 27.4 ms against 29.6-30.2 ms with first_child0).
 constexpr bool zorder(int mode) { return mode != 1; }
 
+// The child to descend into first when both children of a node are
+// interior and hit.  With a last occluder `hint` (leaf-order index, -1:
+// none) and the node's leaf split (child 0's leaves end at `split`, -1:
+// unknown): the child whose leaf range holds the hint, or lies on its side
+// in leaf order.  A search's next query goes a step up or down from the last
+// one, and its occluder is often that triangle or one near it (C2: walk
+// trace 30.4 -> 29.7 ms, certified climb 27.3 -> 25.2-26.3 ms).  Otherwise
+// first_child0's choice (ZORD) or the nearer child.
+__device__ __forceinline__ bool first_child(const Ray& r, int split, int hint, float ztop0,
+                                            float ztop1, float tmin0, float tmax0, float tmin1,
+                                            float tmax1, bool zord) {
+    if (hint >= 0 && split >= 0) return hint <= split;
+    return zord ? first_child0(r, ztop0, ztop1, tmin0, tmax0, tmin1, tmax1) : tmin0 <= tmin1;
+}
+
 // One step of the any-hit traversal: fetch node `node`,
 // test both child boxes, test the triangles of hit leaf children at once,
-// then descend (with both children hit: into first_child0's choice with ZORD,
-// else the nearer one) or pop.  Returns 0 while the query is open, 1 on a hit, 2
+// then descend (with both children hit: into first_child's choice) or pop.  Returns 0 while the query is open, 1 on a hit, 2
 // when the stack empties without one.  The stack keeps its first kLdsStack
 // entries in LDS at stk[k * BLOCK + lane] and the rest in the private array ovf.
 template <bool COUNT, int BLOCK, bool MULTI, bool ZORD = true>
@@ -317,8 +331,8 @@ __device__ __forceinline__ int trav_step(const BvhNode* __restrict__ nodes,
         h1 = h1 && r1 >= 0;
     }
     if (h0 && h1) {
-        const bool first0 =
-            ZORD ? first_child0(r, z0h, z1h, tmin0, tmax0, tmin1, tmax1) : tmin0 <= tmin1;
+        const bool first0 = first_child(r, __float_as_int(d.w), hit_leaf, z0h, z1h, tmin0,
+                                        tmax0, tmin1, tmax1, ZORD);
         int nearc = first0 ? r0 : r1;
         int farc = first0 ? r1 : r0;
         if (sp >= kStackMax) {
@@ -411,8 +425,8 @@ __device__ __forceinline__ int trav_step_pf(const BvhNode* __restrict__ nodes, i
     bool empty = false, over = false;
     int next = 0;
     if (i0 && i1) {
-        const bool first0 =
-            ZORD ? first_child0(r, z0h, z1h, tmin0, tmax0, tmin1, tmax1) : tmin0 <= tmin1;
+        const bool first0 = first_child(r, __float_as_int(d.w), hit_leaf, z0h, z1h, tmin0,
+                                        tmax0, tmin1, tmax1, ZORD);
         const int farc = first0 ? r1 : r0;
         next = first0 ? r0 : r1;
         if (sp >= kStackMax) {

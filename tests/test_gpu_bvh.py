@@ -7,6 +7,8 @@ four-wide collapse), read back with hz_domain_bvh and walked on the host:
   decoded exactly in FP64) contains all vertices of the child's subtree;
 * inline triangles equal the leaf-order array, and the "wide" bit on a
   reference says exactly whether the node holds inline triangles;
+* a binary node with two interior children stores the leaf-order split of
+  its children (-1 only in a top rebuilt with the SAH option);
 * the leaf order is a permutation of the mesh's triangles, with their
   vertices (hz_scene_export) unchanged;
 * all of this also holds with the top of the binary tree rebuilt by the
@@ -46,22 +48,25 @@ def _leaf_tris(tris):
     return v, ids
 
 
-def _check_binary(nodes_raw, tris):
+def _check_binary(nodes_raw, tris, sah=False):
     nodes = nodes_raw.view(BIN_RAW).reshape(-1)
     v, _ = _leaf_tris(tris)
     seen = np.zeros(len(tris), np.int64)
     visits = np.zeros(len(nodes), np.int64)
 
     def walk(i):
+        # returns the subtree's bounds, whether node i holds an inline
+        # triangle, and its leaf-order range when that is contiguous (else None)
         visits[i] += 1
         nd = nodes[i]
         bmin, bmax, inline = np.full(3, np.inf), np.full(3, -np.inf), False
+        ranges = []
         for s in range(2):
             lo = nd["box"][:, s].astype(np.float64)
             hi = nd["box"][:, 2 + s].astype(np.float64)
             r = int(nd["r"][s])
             if r >= 0:
-                cmin, cmax, cin = walk(r & LEAF_MASK)
+                cmin, cmax, cin, crange = walk(r & LEAF_MASK)
                 assert bool(r & WIDE) == cin
             else:
                 code = ~r
@@ -73,9 +78,22 @@ def _check_binary(nodes_raw, tris):
                                           tris[first, :9].view(np.int32))
                 cmin = v[first:first + cnt].min(axis=(0, 1))
                 cmax = v[first:first + cnt].max(axis=(0, 1))
+                crange = (first, first + cnt - 1)
             assert np.all(lo <= cmin) and np.all(hi >= cmax), (i, s)
             bmin, bmax = np.minimum(bmin, cmin), np.maximum(bmax, cmax)
-        return bmin, bmax, inline
+            ranges.append(crange)
+        if int(nd["r"][0]) >= 0 and int(nd["r"][1]) >= 0:
+            # two interior children: the leaf split that orders the traversal
+            split = int(nd["tri"][1:2].view(np.int32)[0])
+            if split >= 0:
+                assert ranges[0] is not None and ranges[1] is not None, i
+                assert ranges[0][1] == split and ranges[1][0] == split + 1, (i, split, ranges)
+            else:
+                assert sah and split == -1, (i, split)
+        whole = None
+        if ranges[0] is not None and ranges[1] is not None and ranges[0][1] + 1 == ranges[1][0]:
+            whole = (ranges[0][0], ranges[1][1])
+        return bmin, bmax, inline, whole
 
     walk(0)
     return seen, visits
@@ -143,7 +161,8 @@ def test_tree_structure(c1_dom, width, leaf, morton, sah):
     nodes, tris = dom.bvh(bvh_width=width, leaf_size=leaf, bvh_morton=morton, bvh_sah=sah)
     ntri = scene["tri"].shape[0]
     assert tris.shape[0] == ntri and nodes.shape[0] >= 1
-    seen, visits = (_check_binary if width == 2 else _check_four)(nodes, tris)
+    seen, visits = (_check_binary(nodes, tris, sah=bool(sah)) if width == 2
+                    else _check_four(nodes, tris))
     assert np.all(seen == 1)
     # no node is reached twice; with one triangle per leaf every node is
     # reached (larger binary leaves leave the nodes below them unreferenced)
