@@ -38,10 +38,16 @@ static_assert(sizeof(CellFrame) == 64, "CellFrame layout");
 //              holds an inline triangle (its second 64 bytes must be read)
 //   ref <  0 : leaf; ~ref = first | ((count - 1) << 28), triangles in leaf
 //              order; a leaf of count 1 is the inline triangle of its slot
+// The z planes of a binary tree's boxes bound z' = z + (x^2 + y^2) * kZCurv,
+// the height above the sphere of radius 1 / (2 kZCurv) that touches the ENU
+// plane at its origin (to second order), not z: terrain far from the origin
+// drops below the ENU plane and tilts against it, and its boxes in z' stay
+// flat (the traversal tests a ray's z' range over its x/y interval).
 // A node whose children are both interior has no inline triangle; its
 // tri[1] then holds (as int bits) the leaf-order index of child 0's last
 // leaf, child 1's leaves following it, or -1 where the children's leaves are
 // not two such ranges (the top rebuilt with the SAH option).
+constexpr double kZCurv = 1.0 / (2.0 * 6371229.0);  // Earth radius, reference :584
 struct alignas(16) BvhNode {
     float lx0, lx1, hx0, hx1;
     float ly0, ly1, hy0, hy1;

@@ -99,6 +99,36 @@ __global__ __launch_bounds__(kBlock) void k_tri_bounds(const float* __restrict__
     }
 }
 
+// Binary trees: replace the z bounds of triangle boxes by bounds of
+// z' = z + (x^2 + y^2) kZCurv.  z' is convex over a triangle, so its maximum
+// is at a vertex; its minimum is at least the lowest vertex z plus the
+// smallest (x^2 + y^2) kZCurv over the triangle's x/y box.  FP64, rounded
+// outward; k_pack grows them like the other planes.  Runs after the Morton
+// codes (which keep the ENU centroids).
+__global__ __launch_bounds__(kBlock) void k_tri_zcurv(const float* __restrict__ vert,
+                                                     const int* __restrict__ tri, int n,
+                                                     float4* __restrict__ lo,
+                                                     float4* __restrict__ hi) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    double zl = 1e300, zh = -1e300, xl = 1e300, xh = -1e300, yl = 1e300, yh = -1e300;
+    for (int k = 0; k < 3; k++) {
+        const int v = tri[3 * i + k];
+        const double x = vert[3 * v], y = vert[3 * v + 1], z = vert[3 * v + 2];
+        zl = fmin(zl, z);
+        zh = fmax(zh, z + (x * x + y * y) * kZCurv);
+        xl = fmin(xl, x);
+        xh = fmax(xh, x);
+        yl = fmin(yl, y);
+        yh = fmax(yh, y);
+    }
+    const double nx = xl > 0.0 ? xl : (xh < 0.0 ? -xh : 0.0);  // nearest |x| on the box
+    const double ny = yl > 0.0 ? yl : (yh < 0.0 ? -yh : 0.0);
+    zl += (nx * nx + ny * ny) * kZCurv;
+    lo[i].z = __double2float_rd(zl);
+    hi[i].z = __double2float_ru(zh);
+}
+
 // One workgroup: reduce nb partials into the encoded bounds b[0..5].
 __global__ __launch_bounds__(kBlock) void k_reduce_bounds(const float* __restrict__ part, int nb,
                                                          unsigned* __restrict__ b) {
@@ -1069,6 +1099,16 @@ int lbvh_build(LbvhWork& w, const float* vert, const int* tri, int n, int leaf_s
     hipLaunchKernelGGL(k_tri_bounds, dim3(grid(n)), dim3(kBlock), 0, st, vert, tri, n, w.aabb_lo,
                        w.aabb_hi, w.part);
     hipLaunchKernelGGL(k_reduce_bounds, dim3(1), dim3(kBlock), 0, st, w.part, grid(n), w.bounds);
+    if (out.width == 2) {
+        // binary trees bound z' in z (after the Morton codes below, which read
+        // the ENU centroids: the codes of n == 1 are not needed for its node)
+        if (n > 1) {
+            hipLaunchKernelGGL(k_morton, dim3(grid(n)), dim3(kBlock), 0, st, w.aabb_lo, w.aabb_hi,
+                               n, w.bounds, morton_mode, w.keys[0], w.vals[0]);
+        }
+        hipLaunchKernelGGL(k_tri_zcurv, dim3(grid(n)), dim3(kBlock), 0, st, vert, tri, n,
+                           w.aabb_lo, w.aabb_hi);
+    }
     if (n == 1) {
         if (out.width == 4) {
             hipLaunchKernelGGL(k_pack4_single, dim3(1), dim3(64), 0, st, w.aabb_lo, w.aabb_hi,
@@ -1087,8 +1127,9 @@ int lbvh_build(LbvhWork& w, const float* vert, const int* tri, int n, int leaf_s
         HZ_HIP(hipGetLastError(), err);
         return 0;
     }
-    hipLaunchKernelGGL(k_morton, dim3(grid(n)), dim3(kBlock), 0, st, w.aabb_lo, w.aabb_hi, n,
-                       w.bounds, morton_mode, w.keys[0], w.vals[0]);
+    if (out.width == 4)
+        hipLaunchKernelGGL(k_morton, dim3(grid(n)), dim3(kBlock), 0, st, w.aabb_lo, w.aabb_hi, n,
+                           w.bounds, morton_mode, w.keys[0], w.vals[0]);
     size_t tmp = w.sort_tmp_bytes;
     HZ_HIP(rocprim::radix_sort_pairs(w.sort_tmp, tmp, w.keys[0], w.keys[1], w.vals[0], w.vals[1],
                                      (unsigned)n, 0, 63, st),

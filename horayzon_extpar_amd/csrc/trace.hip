@@ -115,12 +115,14 @@ __device__ __forceinline__ float sel3(float a, float b, float c, int k) {
 
 // Per-ray constants of the watertight test and of the slab test.
 struct Ray {
-    float okx, oky, okz;  // origin, permuted
     float o0, o1, o2;
     float Sx, Sy, Sz;
-    int kx, ky, kz;
+    int kk;  // the watertight test's axis permutation: kx | ky << 2 | kz << 4
     float ix, iy, iz;     // 1 / direction (zero components replaced by +-1e-20)
     float oix, oiy, oiz;  // origin * (1 / direction)
+    // z' = z + (x^2 + y^2) kZCurv along the ray (the binary trees' z planes):
+    // qa t^2 + qb t + qc, lowest at t = tv with value zv
+    float qa, qb, qc, tv, zv;
     float tfar, tfar_box;
 };
 
@@ -143,15 +145,10 @@ __device__ __forceinline__ Ray make_ray(float o0, float o1, float o2, float d0, 
     r.Sx = dkx / dkz;
     r.Sy = dky / dkz;
     r.Sz = 1.0f / dkz;
-    r.kx = kx;
-    r.ky = ky;
-    r.kz = kz;
+    r.kk = kx | ky << 2 | kz << 4;
     r.o0 = o0;
     r.o1 = o1;
     r.o2 = o2;
-    r.okx = sel3(o0, o1, o2, kx);
-    r.oky = sel3(o0, o1, o2, ky);
-    r.okz = sel3(o0, o1, o2, kz);
     float e0 = fabsf(d0) < 1e-20f ? copysignf(1e-20f, d0) : d0;
     float e1 = fabsf(d1) < 1e-20f ? copysignf(1e-20f, d1) : d1;
     float e2 = fabsf(d2) < 1e-20f ? copysignf(1e-20f, d2) : d2;
@@ -164,6 +161,19 @@ __device__ __forceinline__ Ray make_ray(float o0, float o1, float o2, float d0, 
     r.oix = o0 * r.ix;
     r.oiy = o1 * r.iy;
     r.oiz = o2 * r.iz;
+    // FP32 is enough: the z' planes carry the boxes' margin of at least 1 m,
+    // far above the rounding of these terms (mm over +-400 km)
+    const float kc = (float)kZCurv;
+    r.qa = (d0 * d0 + d1 * d1) * kc;
+    r.qb = d2 + 2.0f * (o0 * d0 + o1 * d1) * kc;
+    r.qc = o2 + (o0 * o0 + o1 * o1) * kc;
+    // the vertex: the hardware reciprocal's 1-ulp error moves zv by far less
+    // than a millimetre wherever tv lies inside a box's interval (|qb| is
+    // then below 2 qa tfar)
+    const bool vtx = r.qa > 0.0f;  // a vertical ray: z' linear, no vertex
+    const float h = __builtin_amdgcn_rcpf(2.0f * r.qa);
+    r.tv = vtx ? -r.qb * h : -3.0e38f;
+    r.zv = vtx ? __builtin_fmaf(0.5f * r.qb, r.tv, r.qc) : 0.0f;
     r.tfar = tfar;  // uniform kernel arguments: scalar registers
     r.tfar_box = tfar_box;
     return r;
@@ -175,9 +185,10 @@ __device__ __forceinline__ bool tri_hit(const Ray& r, float v0x, float v0y, floa
     float A0 = v0x - r.o0, A1 = v0y - r.o1, A2 = v0z - r.o2;
     float B0 = v1x - r.o0, B1 = v1y - r.o1, B2 = v1z - r.o2;
     float C0 = v2x - r.o0, C1 = v2y - r.o1, C2 = v2z - r.o2;
-    float Akx = sel3(A0, A1, A2, r.kx), Aky = sel3(A0, A1, A2, r.ky), Akz = sel3(A0, A1, A2, r.kz);
-    float Bkx = sel3(B0, B1, B2, r.kx), Bky = sel3(B0, B1, B2, r.ky), Bkz = sel3(B0, B1, B2, r.kz);
-    float Ckx = sel3(C0, C1, C2, r.kx), Cky = sel3(C0, C1, C2, r.ky), Ckz = sel3(C0, C1, C2, r.kz);
+    const int kx = r.kk & 3, ky = (r.kk >> 2) & 3, kz = r.kk >> 4;
+    float Akx = sel3(A0, A1, A2, kx), Aky = sel3(A0, A1, A2, ky), Akz = sel3(A0, A1, A2, kz);
+    float Bkx = sel3(B0, B1, B2, kx), Bky = sel3(B0, B1, B2, ky), Bkz = sel3(B0, B1, B2, kz);
+    float Ckx = sel3(C0, C1, C2, kx), Cky = sel3(C0, C1, C2, ky), Ckz = sel3(C0, C1, C2, kz);
     float Ax = Akx - r.Sx * Akz;
     float Ay = Aky - r.Sy * Akz;
     float Bx = Bkx - r.Sx * Bkz;
@@ -222,28 +233,44 @@ __device__ __forceinline__ bool leaf_hit(const Ray& r, const float4* __restrict_
     return false;
 }
 
-// Which hit interior child to descend into first when both are hit: the one
-// whose box top rises furthest above the ray where the ray is lowest inside
-// the box (its entry for a rising ray, its exit for a falling one).  For
-// near-horizontal rays over terrain that box holds an occluder more often
-// than the nearer one: on C2 the walk fetches 5% fewer nodes and tests 12%
-// fewer triangles per query (46.8 and 9.0 against 49.2 and 10.3).  Only the order changes: an any-hit answer is
-// the OR over the triangles either way.  ztop0 / ztop1 are the ray
-// parameters of the two boxes' top planes, (hz - oz) / dz.
-__device__ __forceinline__ bool first_child0(const Ray& r, float ztop0, float ztop1, float tmin0,
-                                             float tmax0, float tmin1, float tmax1) {
-    const bool up = r.iz >= 0.0f;
-    // (ztop - t_low) * dz is the height of the top above the ray there
-    const float g0 = ztop0 - (up ? tmin0 : tmax0), g1 = ztop1 - (up ? tmin1 : tmax1);
-    return up ? g0 >= g1 : g0 <= g1;
+// The two child boxes of a binary node (a = x planes, b = y planes, c = z'
+// planes: lo0 lo1 hi0 hi1 each) against the ray: x / y slabs give the
+// ray's interval [ta, tb] in the box's x/y footprint (within [-1, tfar_box]);
+// over it the ray's z' is convex in t, so its range is spanned by the two
+// ends and the vertex tv when inside; the box is hit when that range meets
+// the box's z' range.  ta is the entry (for the nearer-first order) and g
+// the height of the box top above the ray's lowest point in the box.
+__device__ __forceinline__ void two_boxes(const Ray& r, const float4& a, const float4& b,
+                                          const float4& c, bool& h0, bool& h1, float& ta0,
+                                          float& ta1, float& g0, float& g1) {
+    const float x0l = __builtin_fmaf(a.x, r.ix, -r.oix), x0h = __builtin_fmaf(a.z, r.ix, -r.oix);
+    const float x1l = __builtin_fmaf(a.y, r.ix, -r.oix), x1h = __builtin_fmaf(a.w, r.ix, -r.oix);
+    const float y0l = __builtin_fmaf(b.x, r.iy, -r.oiy), y0h = __builtin_fmaf(b.z, r.iy, -r.oiy);
+    const float y1l = __builtin_fmaf(b.y, r.iy, -r.oiy), y1h = __builtin_fmaf(b.w, r.iy, -r.oiy);
+    ta0 = fmaxf(fmaxf(fminf(x0l, x0h), fminf(y0l, y0h)), -1.0f);
+    ta1 = fmaxf(fmaxf(fminf(x1l, x1h), fminf(y1l, y1h)), -1.0f);
+    const float tb0 = fminf(fminf(fmaxf(x0l, x0h), fmaxf(y0l, y0h)), r.tfar_box);
+    const float tb1 = fminf(fminf(fmaxf(x1l, x1h), fmaxf(y1l, y1h)), r.tfar_box);
+    const float za0 = __builtin_fmaf(__builtin_fmaf(r.qa, ta0, r.qb), ta0, r.qc);
+    const float zb0 = __builtin_fmaf(__builtin_fmaf(r.qa, tb0, r.qb), tb0, r.qc);
+    const float za1 = __builtin_fmaf(__builtin_fmaf(r.qa, ta1, r.qb), ta1, r.qc);
+    const float zb1 = __builtin_fmaf(__builtin_fmaf(r.qa, tb1, r.qb), tb1, r.qc);
+    float lo0 = fminf(za0, zb0), lo1 = fminf(za1, zb1);
+    const float hi0 = fmaxf(za0, zb0), hi1 = fmaxf(za1, zb1);
+    if (r.tv > ta0 && r.tv < tb0) lo0 = fminf(lo0, r.zv);
+    if (r.tv > ta1 && r.tv < tb1) lo1 = fminf(lo1, r.zv);
+    h0 = ta0 <= tb0 && lo0 <= c.z && hi0 >= c.x;
+    h1 = ta1 <= tb1 && lo1 <= c.w && hi1 >= c.y;
+    g0 = c.z - lo0;
+    g1 = c.w - lo1;
 }
 
-// The searches that take first_child0's order: the walk and the learned
+// The searches that take the taller-box order below: the walk and the learned
 // start, whose queries sit near the horizon, where distant high ground
 // occludes (C2: walk trace 31.6 -> 30.4 ms, uncertified learned items
 // 16.7 -> 15.9 ms).  The climb's queries start at 0 deg and mostly hit the
 // ground nearby, where the nearer child first is better (certified climb
-// 27.4 ms against 29.6-30.2 ms with first_child0).
+// 27.4 ms against 29.6-30.2 ms with the taller-box order).
 constexpr bool zorder(int mode) { return mode != 1; }
 
 // The child to descend into first when both children of a node are
@@ -252,13 +279,16 @@ constexpr bool zorder(int mode) { return mode != 1; }
 // unknown): the child whose leaf range holds the hint, or lies on its side
 // in leaf order.  A search's next query goes a step up or down from the last
 // one, and its occluder is often that triangle or one near it (C2: walk
-// trace 30.4 -> 29.7 ms, certified climb 27.3 -> 25.2-26.3 ms).  Otherwise
-// first_child0's choice (ZORD) or the nearer child.
-__device__ __forceinline__ bool first_child(const Ray& r, int split, int hint, float ztop0,
-                                            float ztop1, float tmin0, float tmax0, float tmin1,
-                                            float tmax1, bool zord) {
+// trace 30.4 -> 29.7 ms, certified climb 27.3 -> 25.2-26.3 ms).  Otherwise,
+// with ZORD, the box whose top rises higher above the ray where the ray is
+// lowest inside it (g): for near-horizontal rays over terrain that box holds
+// an occluder more often than the nearer one (C2: walk trace 31.6 -> 30.4
+// ms); else the nearer box.  Only the order changes: an any-hit answer is
+// the OR over the triangles either way.
+__device__ __forceinline__ bool first_child(int split, int hint, float ta0, float ta1, float g0,
+                                            float g1, bool zord) {
     if (hint >= 0 && split >= 0) return hint <= split;
-    return zord ? first_child0(r, ztop0, ztop1, tmin0, tmax0, tmin1, tmax1) : tmin0 <= tmin1;
+    return zord ? g0 >= g1 : ta0 <= ta1;
 }
 
 // One step of the any-hit traversal: fetch node `node`,
@@ -285,18 +315,9 @@ __device__ __forceinline__ int trav_step(const BvhNode* __restrict__ nodes,
     }
     if (COUNT) nvisit++;
     // child boxes: a = (lx0, lx1, hx0, hx1), b = (ly0, ly1, hy0, hy1), c = (lz0, lz1, hz0, hz1)
-    float x0l = __builtin_fmaf(a.x, r.ix, -r.oix), x0h = __builtin_fmaf(a.z, r.ix, -r.oix);
-    float x1l = __builtin_fmaf(a.y, r.ix, -r.oix), x1h = __builtin_fmaf(a.w, r.ix, -r.oix);
-    float y0l = __builtin_fmaf(b.x, r.iy, -r.oiy), y0h = __builtin_fmaf(b.z, r.iy, -r.oiy);
-    float y1l = __builtin_fmaf(b.y, r.iy, -r.oiy), y1h = __builtin_fmaf(b.w, r.iy, -r.oiy);
-    float z0l = __builtin_fmaf(c.x, r.iz, -r.oiz), z0h = __builtin_fmaf(c.z, r.iz, -r.oiz);
-    float z1l = __builtin_fmaf(c.y, r.iz, -r.oiz), z1h = __builtin_fmaf(c.w, r.iz, -r.oiz);
-    float tmin0 = fmaxf(fmaxf(fminf(x0l, x0h), fminf(y0l, y0h)), fmaxf(fminf(z0l, z0h), -1.0f));
-    float tmax0 = fminf(fminf(fmaxf(x0l, x0h), fmaxf(y0l, y0h)), fminf(fmaxf(z0l, z0h), r.tfar_box));
-    float tmin1 = fmaxf(fmaxf(fminf(x1l, x1h), fminf(y1l, y1h)), fmaxf(fminf(z1l, z1h), -1.0f));
-    float tmax1 = fminf(fminf(fmaxf(x1l, x1h), fmaxf(y1l, y1h)), fminf(fmaxf(z1l, z1h), r.tfar_box));
-    bool h0 = tmin0 <= tmax0;
-    bool h1 = tmin1 <= tmax1;
+    float tmin0, tmin1, g0, g1;
+    bool h0, h1;
+    two_boxes(r, a, b, c, h0, h1, tmin0, tmin1, g0, g1);
     const int r0 = __float_as_int(d.x), r1 = __float_as_int(d.y);
     // leaf children: single triangles are inline (slot 0: d.zw e4 f4.xyz,
     // slot 1: f4.w g4 h4); larger leaves (MULTI only) live in the tris array.
@@ -331,8 +352,7 @@ __device__ __forceinline__ int trav_step(const BvhNode* __restrict__ nodes,
         h1 = h1 && r1 >= 0;
     }
     if (h0 && h1) {
-        const bool first0 = first_child(r, __float_as_int(d.w), hit_leaf, z0h, z1h, tmin0,
-                                        tmax0, tmin1, tmax1, ZORD);
+        const bool first0 = first_child(__float_as_int(d.w), hit_leaf, tmin0, tmin1, g0, g1, ZORD);
         int nearc = first0 ? r0 : r1;
         int farc = first0 ? r1 : r0;
         if (sp >= kStackMax) {
@@ -405,18 +425,9 @@ __device__ __forceinline__ int trav_step_pf(const BvhNode* __restrict__ nodes, i
         g4 = np[6];
         h4 = np[7];
     }
-    float x0l = __builtin_fmaf(a.x, r.ix, -r.oix), x0h = __builtin_fmaf(a.z, r.ix, -r.oix);
-    float x1l = __builtin_fmaf(a.y, r.ix, -r.oix), x1h = __builtin_fmaf(a.w, r.ix, -r.oix);
-    float y0l = __builtin_fmaf(b.x, r.iy, -r.oiy), y0h = __builtin_fmaf(b.z, r.iy, -r.oiy);
-    float y1l = __builtin_fmaf(b.y, r.iy, -r.oiy), y1h = __builtin_fmaf(b.w, r.iy, -r.oiy);
-    float z0l = __builtin_fmaf(c.x, r.iz, -r.oiz), z0h = __builtin_fmaf(c.z, r.iz, -r.oiz);
-    float z1l = __builtin_fmaf(c.y, r.iz, -r.oiz), z1h = __builtin_fmaf(c.w, r.iz, -r.oiz);
-    float tmin0 = fmaxf(fmaxf(fminf(x0l, x0h), fminf(y0l, y0h)), fmaxf(fminf(z0l, z0h), -1.0f));
-    float tmax0 = fminf(fminf(fmaxf(x0l, x0h), fmaxf(y0l, y0h)), fminf(fmaxf(z0l, z0h), r.tfar_box));
-    float tmin1 = fmaxf(fmaxf(fminf(x1l, x1h), fminf(y1l, y1h)), fmaxf(fminf(z1l, z1h), -1.0f));
-    float tmax1 = fminf(fminf(fmaxf(x1l, x1h), fmaxf(y1l, y1h)), fminf(fmaxf(z1l, z1h), r.tfar_box));
-    const bool h0 = tmin0 <= tmax0;
-    const bool h1 = tmin1 <= tmax1;
+    float tmin0, tmin1, g0, g1;
+    bool h0, h1;
+    two_boxes(r, a, b, c, h0, h1, tmin0, tmin1, g0, g1);
     const int r0 = __float_as_int(d.x), r1 = __float_as_int(d.y);
     const float t0 = d.z, t1 = d.w;                     // slot 0's first two coordinates
     const bool l0 = h0 && r0 < 0, l1 = h1 && r1 < 0;    // hit leaf children
@@ -425,8 +436,7 @@ __device__ __forceinline__ int trav_step_pf(const BvhNode* __restrict__ nodes, i
     bool empty = false, over = false;
     int next = 0;
     if (i0 && i1) {
-        const bool first0 = first_child(r, __float_as_int(d.w), hit_leaf, z0h, z1h, tmin0,
-                                        tmax0, tmin1, tmax1, ZORD);
+        const bool first0 = first_child(__float_as_int(d.w), hit_leaf, tmin0, tmin1, g0, g1, ZORD);
         const int farc = first0 ? r1 : r0;
         next = first0 ? r0 : r1;
         if (sp >= kStackMax) {

@@ -312,8 +312,11 @@ int hz_domain_scene(const hz_domain* dom, float* vert32, int* tri, double* frame
  * nodes[128 * num_nodes] bytes of BvhNode (width 2) or Bvh4Node (width 4)
  * records, root first, and tris[12 * num_tris] floats, the triangles in
  * leaf order (v0 xyz, v1 xyz, v2 xyz, original index as int bits, 2 pad).
- * NULL arrays are skipped (call once for the counts).  For checking the
- * tree's structure; the reference's counterpart is Embree's internal build
+ * NULL arrays are skipped (call once for the counts).  A binary node's z
+ * planes bound z' = z + (x^2 + y^2) / (2 * 6371229), the height above the
+ * sphere touching the ENU plane at its origin, and a node whose children are
+ * both interior holds their leaf-order split as int bits in tri[1].  For
+ * checking the tree's structure; the reference's counterpart is Embree's internal build
  * (rtcCommitScene, mo_lradtopo_horayzon.cpp:422), which it cannot read back.
  */
 int hz_domain_bvh(hz_domain* dom, const hz_options* opt, void* nodes, long long* num_nodes,

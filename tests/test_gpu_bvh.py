@@ -3,8 +3,9 @@ four-wide collapse), read back with hz_domain_bvh and walked on the host:
 
 * every triangle is reached exactly once, no node twice (and every node
   once where the tree has no collapsed leaves);
-* every child box (binary: FP32 planes; four-wide: the 8-bit planes
-  decoded exactly in FP64) contains all vertices of the child's subtree;
+* every child box (binary: FP32 planes, z' = z + (x^2 + y^2) / 2R in the z
+  planes; four-wide: the 8-bit planes decoded exactly in FP64) contains all
+  vertices of the child's subtree;
 * inline triangles equal the leaf-order array, and the "wide" bit on a
   reference says exactly whether the node holds inline triangles;
 * a binary node with two interior children stores the leaf-order split of
@@ -48,9 +49,15 @@ def _leaf_tris(tris):
     return v, ids
 
 
+# binary trees bound z' = z + (x^2 + y^2) * KZCURV in their z planes
+KZCURV = 1.0 / (2.0 * 6371229.0)
+
+
 def _check_binary(nodes_raw, tris, sah=False):
     nodes = nodes_raw.view(BIN_RAW).reshape(-1)
     v, _ = _leaf_tris(tris)
+    v = v.copy()
+    v[:, :, 2] += (v[:, :, 0] ** 2 + v[:, :, 1] ** 2) * KZCURV
     seen = np.zeros(len(tris), np.int64)
     visits = np.zeros(len(nodes), np.int64)
 
