@@ -101,8 +101,11 @@ __global__ __launch_bounds__(kBlock) void k_tri_bounds(const float* __restrict__
 
 // Binary trees: replace the z bounds of triangle boxes by bounds of
 // z' = z + (x^2 + y^2) kZCurv.  z' is convex over a triangle, so its maximum
-// is at a vertex; its minimum is at least the lowest vertex z plus the
-// smallest (x^2 + y^2) kZCurv over the triangle's x/y box.  FP64, rounded
+// is at a vertex.  Below its linear interpolation between the vertices it
+// dips by kZCurv times the barycentric variance of the vertices' x/y
+// positions, at most diam^2 / 3 (an equilateral triangle's circumradius
+// squared), so the lowest vertex z' minus kZCurv diam^2 / 3 bounds it from
+// below, diam^2 taken as the x/y box's diagonal squared.  FP64, rounded
 // outward; k_pack grows them like the other planes.  Runs after the Morton
 // codes (which keep the ENU centroids).
 __global__ __launch_bounds__(kBlock) void k_tri_zcurv(const float* __restrict__ vert,
@@ -115,16 +118,15 @@ __global__ __launch_bounds__(kBlock) void k_tri_zcurv(const float* __restrict__ 
     for (int k = 0; k < 3; k++) {
         const int v = tri[3 * i + k];
         const double x = vert[3 * v], y = vert[3 * v + 1], z = vert[3 * v + 2];
-        zl = fmin(zl, z);
-        zh = fmax(zh, z + (x * x + y * y) * kZCurv);
+        const double zc = z + (x * x + y * y) * kZCurv;
+        zl = fmin(zl, zc);
+        zh = fmax(zh, zc);
         xl = fmin(xl, x);
         xh = fmax(xh, x);
         yl = fmin(yl, y);
         yh = fmax(yh, y);
     }
-    const double nx = xl > 0.0 ? xl : (xh < 0.0 ? -xh : 0.0);  // nearest |x| on the box
-    const double ny = yl > 0.0 ? yl : (yh < 0.0 ? -yh : 0.0);
-    zl += (nx * nx + ny * ny) * kZCurv;
+    zl -= ((xh - xl) * (xh - xl) + (yh - yl) * (yh - yl)) * kZCurv / 3.0;
     lo[i].z = __double2float_rd(zl);
     hi[i].z = __double2float_ru(zh);
 }
