@@ -107,6 +107,7 @@ struct hz_domain {
     int* d_tri = nullptr;
     CellFrame* d_cells = nullptr;
     int* d_order = nullptr;
+    int* d_start = nullptr;  // per cell: the node its queries start at (binary trees)
     BvhNode* d_nodes = nullptr;
     float4* d_tris = nullptr;
     double* d_horizon = nullptr;
@@ -247,7 +248,7 @@ void hz_domain_destroy(hz_domain* d) {
                     d->d_spill,  d->d_elev_cs, d->d_desc,    d->d_pred,
                     d->d_adj_off, d->d_adj,    d->d_slope,   d->d_hitem,
                     d->d_state2, d->d_state3, d->d_ranges, d->d_spill_x,
-                    d->d_pcell,  d->d_holes};
+                    d->d_pcell,  d->d_holes,  d->d_start};
     d->stager.release();
     for (void* b : bufs)
         if (b) (void)hipFree(b);
@@ -775,6 +776,12 @@ int hz_domain_run(hz_domain* d, int num_hori, double dist_search, int refine_fac
         p.pred_stride = kOutPad;
     }
     RCHK(hipEventRecord(d->ev[4], st));
+    if (width == 2) {  // queries start below the levels whose other child is out of reach
+        if (!d->d_start) RCHK(hipMalloc(&d->d_start, (size_t)d->ncell * sizeof(int)));
+        launch_start_nodes(d->bvh, d->d_cells, d->ncell, p.tfar_box + 1.0f, d->d_start, st);
+        RCHK(hipGetLastError());
+        p.start = d->d_start;
+    }
     bool multi_state = false;  // the border walk and far items keep states of their own
     float ms_cert = 0, ms_cwalk = 0, ms_bwalk = 0, ms_lattice = 0;
     if (!cert_run) {

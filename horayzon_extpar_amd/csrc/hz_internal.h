@@ -133,6 +133,9 @@ struct WalkParams {
     // device {first, count}: the item kernels, k_items_finish and k_certify
     // work on these positions of the trace order only (NULL: all cells)
     const int* range;
+    // binary trees: per cell, the reference of the node its queries start
+    // at (launch_start_nodes; NULL: the root)
+    const int* start;
 };
 
 constexpr int kMaxLattice = 180;  // 90 deg
@@ -439,6 +442,13 @@ int batch_grid(int mode, int block, bool multi, int width, int total);
 // hold trace_spill_doubles(ncell * num_hori * refine_factor) doubles.
 // items_finish: with hitem, form the sector means and SVF from the item
 // results (k_items_finish); false when the certificate does it instead.
+// Per cell, the node its queries can start at in the binary tree bvh: from
+// the root, the child whose x/y box holds the cell's ray origin while the
+// other child's x/y box lies farther than reach (no ray of length tfar from
+// the origin can reach a triangle there); the node's reference (index and
+// wide bit) goes to start[cell].
+void launch_start_nodes(const DeviceBvh& bvh, const CellFrame* cells, int ncell, float reach,
+                        int* start, hipStream_t stream);
 int launch_trace(const DeviceBvh& bvh, const CellFrame* cells, const int* order,
                  const WalkParams& p, int mode, int kernel, int block_size, int refill,
                  bool counters, double* horizon, double* skyview, TraceState* state,

@@ -149,6 +149,7 @@ WalkParams make_walk_params(int num_hori, int refine_factor, int svf_type, doubl
     p.sin_h = nullptr;
     p.reserve_blocks = 0;
     p.range = nullptr;
+    p.start = nullptr;
     return p;
 }
 

@@ -1224,6 +1224,9 @@ __global__ __launch_bounds__(BLOCK, 4) void k_trace_pt(const BvhNode* __restrict
                 if (issue) {
                     SI(S_I0) = make_int2(cell, phase);
                     node = kWideBit;  // the root is read in full
+                    if constexpr (TREE != 2 && !BATCH) {
+                        if (P.start) node = P.start[cell];  // below the levels out of reach
+                    }
                     sp = 0;
                     steps = 0;
                     live = true;
@@ -1791,6 +1794,45 @@ int launch_walk_team(const DeviceBvh& bvh, const CellFrame* cells, const int* or
                          grid_blocks, st);
     HZ_HIP(hipGetLastError(), err);
     return 0;
+}
+
+namespace {
+
+__global__ __launch_bounds__(256) void k_start_nodes(const BvhNode* __restrict__ nodes,
+                                                    const CellFrame* __restrict__ cells,
+                                                    int ncell, float reach,
+                                                    int* __restrict__ start) {
+    const int c = blockIdx.x * 256 + threadIdx.x;
+    if (c >= ncell) return;
+    const float ox = cells[c].org[0], oy = cells[c].org[1];
+    const float r2 = reach * reach;
+    int ref = kWideBit;  // the root
+    for (int depth = 0; depth < 128; depth++) {  // a chain of interior nodes: < 128 levels
+        const float4* np = reinterpret_cast<const float4*>(nodes + (ref & kLeafFirstMask));
+        const float4 a = np[0], b = np[1], d = np[3];  // x planes, y planes, references
+        const bool in0 = ox >= a.x && ox <= a.z && oy >= b.x && oy <= b.z;
+        const bool in1 = ox >= a.y && ox <= a.w && oy >= b.y && oy <= b.w;
+        if (in0 == in1) break;  // both children hold the origin, or neither
+        // the other child's x/y box: its distance from the origin
+        const float lx = in0 ? a.y : a.x, hx = in0 ? a.w : a.z;
+        const float ly = in0 ? b.y : b.x, hy = in0 ? b.w : b.z;
+        const float dx = fmaxf(fmaxf(lx - ox, ox - hx), 0.0f);
+        const float dy = fmaxf(fmaxf(ly - oy, oy - hy), 0.0f);
+        if (dx * dx + dy * dy <= r2) break;  // within reach: its subtree stays in
+        const int child = __float_as_int(in0 ? d.x : d.y);
+        if (child < 0) break;  // a leaf
+        ref = child;
+    }
+    start[c] = ref;
+}
+
+}  // namespace
+
+void launch_start_nodes(const DeviceBvh& bvh, const CellFrame* cells, int ncell, float reach,
+                        int* start, hipStream_t stream) {
+    if (ncell <= 0 || bvh.width != 2 || bvh.num_tri <= 0) return;
+    hipLaunchKernelGGL(k_start_nodes, dim3((ncell + 255) / 256), dim3(256), 0, stream, bvh.nodes,
+                       cells, ncell, reach, start);
 }
 
 int launch_trace(const DeviceBvh& bvh, const CellFrame* cells, const int* order,
