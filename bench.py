@@ -159,8 +159,8 @@ def main():
                      "limiter": "the node fetches run at the GPU's rate for dependent "
                                 "random 64-byte gathers (gather_ceiling, measured in this "
                                 "run; it does not rise from 2 to 8 wavefronts per SIMD); "
-                                "symptoms: VALU 67-72% busy at 25% lane use, the vector "
-                                "data path 85% busy with 43% waiting on L1 for node fetches "
+                                "symptoms: VALU 64-65% busy at 24% lane use, the vector "
+                                "data path 81% busy with 44% waiting on L1 for node fetches "
                                 "that missed L2, HBM below peak; the HBM roofline is the "
                                 "bound the kernel is priced against",
                      "gather_ceiling": gather,
