@@ -227,7 +227,7 @@ void hz_default_options(hz_options* o) {
     o->cell_order = 1;
     o->verbose = 0;
     o->trace_kernel = 0;
-    o->refill = 40;
+    o->refill = kDefaultRefill;
     o->queue_chunk = 0;
     o->bvh_width = 2;
     o->prologue = HZ_PROLOGUE_HOST;
@@ -626,7 +626,7 @@ int hz_domain_run(hz_domain* d, int num_hori, double dist_search, int refine_fac
     if (o.bvh_morton < 0 || o.bvh_morton > 1) return fail(HZ_EINVAL, "bvh_morton must be 0 or 1");
     if (o.trace_kernel < 0 || o.trace_kernel > 1)
         return fail(HZ_EINVAL, "trace_kernel must be 0 or 1");
-    int refill = o.refill ? o.refill : 40;
+    int refill = o.refill ? o.refill : kDefaultRefill;
     if (refill < 1 || refill > 64) return fail(HZ_EINVAL, "refill must be in [1, 64]");
     if (o.trace_kernel == 1) width = 2;  // the lane-per-cell kernel walks binary trees
     if (o.certify < 0 || o.certify > 1) return fail(HZ_EINVAL, "certify must be 0 or 1");
@@ -1425,7 +1425,7 @@ int hz_batch_run(hz_batch* b, int num_hori, double dist_search, int refine_facto
         return fail(HZ_EINVAL, "block_size must be 64, 128 or 256");
     int leaf = o.leaf_size ? o.leaf_size : 1;
     if (leaf < 1 || leaf > kMaxLeafSize) return fail(HZ_EINVAL, "leaf_size must be in [1, 8]");
-    int refill = o.refill ? o.refill : 40;
+    int refill = o.refill ? o.refill : kDefaultRefill;
     if (refill < 1 || refill > 64) return fail(HZ_EINVAL, "refill must be in [1, 64]");
     int chunk = o.queue_chunk ? o.queue_chunk : 64;  // cells per queue fetch
     if (chunk < 1 || chunk > 4096) return fail(HZ_EINVAL, "queue_chunk must be in [1, 4096]");

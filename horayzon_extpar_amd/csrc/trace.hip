@@ -1730,7 +1730,7 @@ int batch_grid(int mode, int block, bool multi, int width, int total) {
 int launch_trace_batch(const BatchDomain* doms, int ndom, int total, bool multi, int width,
                        const WalkParams& p, int mode, int block, int refill, int chunk,
                        TraceState* ts, double* spill, hipStream_t st, std::string& err) {
-    if (refill < 1 || refill > 64) refill = 40;
+    if (refill < 1 || refill > 64) refill = kDefaultRefill;
     if (chunk < 1) chunk = kChunk;
     const int tk = tree_kind(multi, width);
     hipLaunchKernelGGL(k_clear_state, dim3(1), dim3(256), 0, st, ts);
@@ -1840,7 +1840,7 @@ int launch_trace(const DeviceBvh& bvh, const CellFrame* cells, const int* order,
                  double* horizon, double* skyview, TraceState* ts, double* spill, double* hitem,
                  bool items_finish, hipStream_t st, std::string& err) {
     const bool multi = bvh.leaf_size > 1;
-    if (refill < 1 || refill > 64) refill = 40;
+    if (refill < 1 || refill > 64) refill = kDefaultRefill;
     hipLaunchKernelGGL(k_clear_state, dim3(1), dim3(256), 0, st, ts);
     if (mode == 2 && (kernel == 1 || !p.pred)) {
         err = "the learned search needs the persistent kernel and predictions";

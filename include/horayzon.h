@@ -46,7 +46,7 @@ typedef struct hz_options {
     int trace_kernel;     /* 0: persistent lanes with per-XCD work queues (default),
                              1: one lane per cell with azimuths in lock step */
     int refill;           /* persistent kernel: refill lanes once fewer than this many of
-                             the 64 lanes of a wavefront hold a live query (1..64; 0 = default) */
+                             the 64 lanes of a wavefront hold a live query (1..64; 0 = default, 36) */
     int bvh_morton;       /* LBVH codes: 0 = 3D in a cube of the largest extent, 1 = 2D x/y */
     int queue_chunk;      /* persistent kernel: most cells a wavefront takes per work-queue
                              fetch (it takes as many as its lanes ask for; 0 = 64) */
